@@ -1,0 +1,213 @@
+/*
+ * lbm_mirror_amd — C ABI of the MI355X (gfx950) lattice Boltzmann engine.
+ *
+ * Drop-in boundary for the reference's component API (keldu/lbm_mirror,
+ * modules/core + modules/sycl).  The reference builds
+ * `component<T, Desc, cmpt::Kind, encode::Sycl>` objects on the host, captures
+ * them by value in a `parallel_for` lambda and calls `apply(field, macros,
+ * index, t)` once per cell (examples/poiseulle_2d_gpu/poiseulle_2d_gpu.cpp:
+ * 158-235).  A per-cell `apply` cannot cross a C ABI, so the same component
+ * kinds and parameters are handed over at field granularity: a table maps
+ * each cell flag to (boundary component, collision component), exactly the
+ * `switch(info)` of the reference drivers, and one call advances whole steps.
+ *
+ * Conventions shared by every entry point
+ *  - Every function returns an lbm_status (0 = ok).  Codes map 1:1 onto the
+ *    reference error categories (modules/core/c++/abstract/error.hpp:209-259).
+ *    No C++ exception crosses this boundary.  lbm_last_error() returns text.
+ *  - Host buffers are owned by the caller, copied during the call, never kept.
+ *  - Cells are x-fastest: cell = x + nx*(y + ny*z), the reference's
+ *    flatten_index order (modules/core/c++/flatten.hpp:11-39) without its
+ *    ghost ring.  The ghost ring (flag 0, populations 0) is implicit.
+ *  - Population arrays at the boundary are SoA doubles f[i*N + cell], i in
+ *    the reference's direction order (descriptor/d2q9.hpp:19-29,
+ *    d3q19.hpp:9-40, d3q27.hpp:17-48).
+ *  - One context owns one HIP stream; calls are stream-ordered and become
+ *    host-visible at the get_* calls or lbm_sync.  A context is not thread
+ *    safe: one host thread per context.
+ */
+#ifndef LBM_MIRROR_AMD_LBM_H
+#define LBM_MIRROR_AMD_LBM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes: abstract/error.hpp:209-259 categories ---------------- */
+typedef enum lbm_status {
+  LBM_OK = 0,
+  LBM_ERR_CRITICAL = 1,       /* err::critical — HIP runtime failure        */
+  LBM_ERR_NOT_FOUND = 2,      /* err::not_found                             */
+  LBM_ERR_OUT_OF_MEMORY = 3,  /* err::out_of_memory — hipMalloc failed      */
+  LBM_ERR_INVALID_STATE = 4,  /* err::invalid_state — call order / NaN      */
+  LBM_ERR_NOT_SUPPORTED = 5,  /* err::not_supported                         */
+  LBM_ERR_INVALID_ARG = 6,    /* err::invalid_argument                      */
+  LBM_ERR_OVERFLOW = 7        /* cell-list bin overflow (cell_list.hpp:99)  */
+} lbm_status;
+
+/* ---- lattices: descriptor/{d2q9,d3q19,d3q27}.hpp ------------------------ */
+typedef enum lbm_lattice { LBM_D2Q9 = 0, LBM_D3Q19 = 1, LBM_D3Q27 = 2 } lbm_lattice;
+
+/* ---- storage / arithmetic -------------------------------------------------
+ * F64, F32: populations stored and computed in T, with the reference's
+ *   per-operation rounding (abstract/data.hpp:170-188) and its FP32
+ *   equilibrium bracket evaluated in double (equilibrium.hpp:36-44).
+ * F16S, BF16S: populations stored as the deviation f_i - w_i in 16 bits,
+ *   arithmetic in FP32 (no reference counterpart; the reference's
+ *   MixedPrecision schema, abstract/schema.hpp:95-112, is the precedent).   */
+typedef enum lbm_storage { LBM_F64 = 0, LBM_F32 = 1, LBM_F16S = 2, LBM_BF16S = 3 } lbm_storage;
+
+/* ---- per-flag operators: the driver switch(info) --------------------------
+ * boundary runs first, then collision, on the pre-collision populations of
+ * the step; every cell whose flag is > 0 is streamed.                      */
+typedef enum lbm_boundary_kind {
+  LBM_BC_NONE = 0,
+  LBM_BC_BOUNCE_BACK = 1,       /* cmpt::BounceBack        boundary.hpp:57-75   */
+  LBM_BC_ANTI_BOUNCE_BACK = 2,  /* cmpt::AntiBounceBack<0> boundary.hpp:106-132 (D2Q9) */
+  LBM_BC_ZOUHE_PRESSURE_X = 3,  /* cmpt::ZouHePressureX<east> boundary.hpp:175-227 (D2Q9), 229-756 (D3Q27) */
+  LBM_BC_ZOUHE_VELOCITY_X = 4,  /* cmpt::ZouHeVelocityX<east> boundary.hpp:759-805 (D2Q9) */
+  LBM_BC_EQUILIBRIUM = 5        /* cmpt::Equilibrium       boundary.hpp:135-161 */
+} lbm_boundary_kind;
+
+typedef enum lbm_collision_kind {
+  LBM_COLL_NONE = 0,
+  LBM_COLL_BGK = 1,      /* cmpt::BGK     collision.hpp:76-96            */
+  LBM_COLL_BGK_GUO = 2,  /* cmpt::BGKGuo  collision.hpp:135-204          */
+  LBM_COLL_PSM = 3       /* cmpt::Psm     psm.hpp:58-89                  */
+} lbm_collision_kind;
+
+/* lbm_op_desc.flags */
+#define LBM_OPF_GUO_CELL_FORCE  0x1u  /* BGKGuo also reads the per-cell force field (collision.hpp:260-263) */
+#define LBM_OPF_PSM_STAMP_DISC  0x2u  /* cmpt::OneParticleAt before Psm (particle/particle_opa.hpp:14-46) */
+#define LBM_OPF_PSM_FORCE_GATHER 0x4u /* cmpt::ForceGather after Psm (momentum_gather.hpp:15-52)         */
+#define LBM_OPF_PSM_CELL_FORCE  0x8u  /* cmpt::PSM variant writing Σ(f_i-f_opp)c_i·B (component/psm.hpp:16-80) */
+
+typedef struct lbm_op_desc {
+  int32_t boundary;      /* lbm_boundary_kind                                   */
+  int32_t collision;     /* lbm_collision_kind                                  */
+  int32_t east;          /* Zou/He template flag, reference meaning: 1 = x=0 face */
+  uint32_t flags;        /* LBM_OPF_*                                           */
+  double tau;            /* relaxation time; ω = T(1)/T(τ) (collision.hpp:25-38) */
+  /* Zou/He pressure set-point, as the drivers compute it per step:
+   *   ρ_set(t) = t < ramp_steps ? rho_start + rho_slope·t : rho_final
+   * (poiseulle_2d_gpu.cpp:182-190).  ramp_steps = 0 means ρ_set = rho_final. */
+  double rho_start, rho_slope, rho_final;
+  uint64_t ramp_steps;
+  double vec[3];         /* BGKGuo global force / Equilibrium velocity          */
+  double rho_eq;         /* Equilibrium density                                 */
+  double disc[5];        /* OneParticleAt: centre x,y,z, radius, eps            */
+} lbm_op_desc;
+
+/* ---- particles: particle/schema.hpp ParticleRigidBody + Spheroid ----------
+ * One record per particle, doubles; 2D uses component 0 of the rotation
+ * fields (the scalar out-of-plane rotation, schema.hpp:22-33).             */
+typedef struct lbm_particle {
+  double position[3], position_old[3];
+  double rotation[3], rotation_old[3];
+  double acceleration[3], angular_acceleration[3];
+  double force[3], torque[3];
+  double mass, inertia, density, radius;
+} lbm_particle;
+
+typedef enum lbm_porosity_mode {
+  LBM_POROSITY_SCATTER = 0,  /* PsmOneParticleGather stamp: each AABB cell takes
+                                the porosity of the highest-index particle whose
+                                AABB holds it (psm.hpp:258-271, serial order)   */
+  LBM_POROSITY_PRODUCT = 1   /* cell_porosity: product over cell-list candidates
+                                (particle/couple.hpp:26-60)                     */
+} lbm_porosity_mode;
+
+typedef struct lbm_particle_params {
+  double k_n, zeta_n, gamma_t, mu;  /* dem_params (particle/contact.hpp:36-61)      */
+  int32_t walls;                    /* ParticleContact with extent (contact.hpp:533-546) */
+  int32_t porosity_mode;            /* lbm_porosity_mode                             */
+  int32_t psm_torque;               /* 0: reference PSM (no torque, psm.hpp:404-443);
+                                       1: add -Σ r×F_cell (hlbm.hpp:234,249-252)    */
+  int32_t cell_capacity;            /* ParticleFluidCouple<Cap> bin capacity         */
+  uint64_t sub_steps;               /* particle sub-steps; 0 means 1 (integrate.hpp:16-19) */
+  double body_acceleration[3];      /* ParticleIntegrate body acceleration           */
+  double eps;                       /* porosity ramp width (psm.hpp:213: 1.5)        */
+  double max_radius;                /* cell-list sizing (cell_list.hpp:185-206)      */
+} lbm_particle_params;
+
+typedef struct lbm_ctx lbm_ctx;
+
+/* ---- context ------------------------------------------------------------- */
+const char* lbm_last_error(void);
+const char* lbm_version(void);
+/* Number of visible GPUs (0 when none); never allocates. */
+int lbm_device_count(void);
+
+lbm_status lbm_ctx_create(lbm_ctx** out, int32_t lattice, int32_t storage,
+                          const uint64_t extent[3], int32_t device);
+lbm_status lbm_ctx_destroy(lbm_ctx* ctx);
+
+/* Cell flags (reference meaning: 0 ghost, 1 wall, 2 fluid, 3+ driver-defined,
+ * geometry.hpp:16-19).  Flag 0 inside the lattice is rejected with
+ * LBM_ERR_NOT_SUPPORTED: the single-buffer AA layout cannot hold the
+ * reference's stale second buffer for such cells. */
+lbm_status lbm_set_flags(lbm_ctx* ctx, const uint8_t* flags);
+/* Periodic wrap per axis (bit 0 = x, 1 = y, 2 = z).  No reference
+ * counterpart: the reference only has zero ghosts (stream.hpp:47). */
+lbm_status lbm_set_periodic(lbm_ctx* ctx, uint32_t axis_mask);
+lbm_status lbm_set_operator(lbm_ctx* ctx, uint8_t flag, const lbm_op_desc* op);
+/* Per-cell velocity for LBM_BC_ZOUHE_VELOCITY_X cells, D doubles per cell,
+ * u[d*N + cell] (the per-cell vel built in stdkr_2d_bgk/step.hpp:44-58). */
+lbm_status lbm_set_boundary_velocity(lbm_ctx* ctx, const double* u);
+/* Per-cell force field ("force" macro), D doubles per cell. */
+lbm_status lbm_set_force_field(lbm_ctx* ctx, const double* force);
+lbm_status lbm_get_force_field(lbm_ctx* ctx, double* force);
+lbm_status lbm_set_porosity(lbm_ctx* ctx, const double* porosity);
+lbm_status lbm_get_porosity(lbm_ctx* ctx, double* porosity);
+
+/* Populations: f[i*N + cell], the pre-collision (post-stream) populations of
+ * the next step, i.e. what the reference holds in dfs_old (t even) or dfs
+ * (t odd) — whatever the physical AA phase is. */
+lbm_status lbm_set_populations(lbm_ctx* ctx, const double* f);
+lbm_status lbm_get_populations(lbm_ctx* ctx, double* f);
+/* Fill every cell with eq(ρ, u) in storage precision (init of the drivers,
+ * poiseulle_2d_gpu.cpp:104-134) and reset macros to (ρ, u). */
+lbm_status lbm_init_equilibrium(lbm_ctx* ctx, double rho, const double u[3]);
+
+/* Macros with the reference's meaning: density/velocity written by the
+ * collision from the pre-collision populations of the last step; wall cells
+ * keep what they held (collision.hpp:83-89, psm.hpp:67-73). */
+lbm_status lbm_get_macros(lbm_ctx* ctx, double* rho, double* u);
+lbm_status lbm_set_macros(lbm_ctx* ctx, const double* rho, const double* u);
+
+/* Advance nsteps reference steps t0 .. t0+nsteps-1.  Without particles one
+ * step is collide+stream (poiseulle_2d_gpu.cpp:342-399); with particles it is
+ * porosity reset, particle reset/gather/contact/integrate, collide, stream
+ * (moving_poiseulle_particles_2d_psm_gpu/sim.cpp:176-280). */
+lbm_status lbm_step(lbm_ctx* ctx, uint64_t t0, uint64_t nsteps);
+lbm_status lbm_sync(lbm_ctx* ctx);
+/* Device time of the last lbm_step call, from HIP events recorded on the
+ * context's stream around its launches. */
+lbm_status lbm_last_step_ms(lbm_ctx* ctx, double* ms);
+
+/* ---- particles ------------------------------------------------------------ */
+lbm_status lbm_particles_set(lbm_ctx* ctx, const lbm_particle* p, uint32_t n,
+                             const lbm_particle_params* params);
+lbm_status lbm_particles_get(lbm_ctx* ctx, lbm_particle* p, uint32_t n);
+/* Disc/sphere mass and inertia exactly as create_spheroid_particle_group
+ * computes them in storage precision (particle/particle.hpp:79-101). */
+lbm_status lbm_spheroid_mass(int32_t dims, int32_t storage, double radius,
+                             double density, double* mass, double* inertia);
+/* One PSM coupling pass at step t on the current populations: porosity field,
+ * per-cell force field and the per-particle FP64 force/torque reduction
+ * (psm.hpp:95-446).  The FP64 sums are read with lbm_particle_reduce. */
+lbm_status lbm_psm_couple(lbm_ctx* ctx, uint64_t t);
+lbm_status lbm_particle_reduce(lbm_ctx* ctx, double* force, double* torque);
+/* DEM contact (contact.hpp:522-567) and integration (integrate.hpp:263-296). */
+lbm_status lbm_dem_contact(lbm_ctx* ctx);
+lbm_status lbm_particle_integrate(lbm_ctx* ctx);
+/* Cell-list overflow count of the last build (cell_list.hpp:96-101). */
+lbm_status lbm_cell_list_overflow(lbm_ctx* ctx, uint64_t* overflow);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

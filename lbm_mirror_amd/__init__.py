@@ -1,0 +1,105 @@
+"""lbm_mirror_amd — MI355X-native lattice Boltzmann engine (gfx950 HIP).
+
+Host-side mirror of the reference's component API (keldu/lbm_mirror,
+modules/core/c++/{collision,boundary,psm,stream}.hpp) over the C ABI declared
+in include/lbm_mirror_amd/lbm.h.  The compute path is the HIP library
+``_build/liblbm_mirror_amd.so``; there is no CPU fallback: if the library is
+missing or no GPU is visible, constructing a lattice raises.
+"""
+import ctypes as C
+import os
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "liblbm_mirror_amd.so")
+
+__all__ = ["abi", "load_library", "LIB_PATH", "LbmError"]
+
+
+class LbmError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"[{abi.STATUS_NAMES.get(status, status)}] {msg}")
+        self.status = status
+
+
+_lib = None
+
+# every symbol include/lbm_mirror_amd/lbm.h declares
+EXPORTS = [
+    "lbm_last_error", "lbm_version", "lbm_device_count", "lbm_ctx_create", "lbm_ctx_destroy",
+    "lbm_set_flags", "lbm_set_periodic", "lbm_set_operator", "lbm_set_boundary_velocity",
+    "lbm_set_force_field", "lbm_get_force_field", "lbm_set_porosity", "lbm_get_porosity",
+    "lbm_set_populations", "lbm_get_populations", "lbm_init_equilibrium", "lbm_get_macros",
+    "lbm_set_macros", "lbm_step", "lbm_sync", "lbm_last_step_ms", "lbm_particles_set",
+    "lbm_particles_get", "lbm_spheroid_mass", "lbm_psm_couple", "lbm_particle_reduce",
+    "lbm_dem_contact", "lbm_particle_integrate", "lbm_cell_list_overflow",
+]
+
+
+def load_library(path=None):
+    """Load liblbm_mirror_amd.so and declare its prototypes.  Raises if the
+    library has not been built (no silent fallback)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(f"HIP library not built: {p} (run __graft_entry__.build())")
+    L = C.CDLL(p)
+    dp = C.POINTER(C.c_double)
+    vp = C.c_void_p
+    st = C.c_int
+    L.lbm_last_error.restype = C.c_char_p
+    L.lbm_version.restype = C.c_char_p
+    L.lbm_device_count.restype = C.c_int
+    L.lbm_ctx_create.argtypes = [C.POINTER(vp), C.c_int32, C.c_int32, C.POINTER(C.c_uint64), C.c_int32]
+    L.lbm_ctx_destroy.argtypes = [vp]
+    L.lbm_set_flags.argtypes = [vp, C.POINTER(C.c_uint8)]
+    L.lbm_set_periodic.argtypes = [vp, C.c_uint32]
+    L.lbm_set_operator.argtypes = [vp, C.c_uint8, C.POINTER(abi.OpDesc)]
+    for name in ("lbm_set_boundary_velocity", "lbm_set_force_field", "lbm_set_porosity",
+                 "lbm_set_populations"):
+        getattr(L, name).argtypes = [vp, dp]
+    for name in ("lbm_get_force_field", "lbm_get_porosity", "lbm_get_populations"):
+        getattr(L, name).argtypes = [vp, dp]
+    L.lbm_init_equilibrium.argtypes = [vp, C.c_double, dp]
+    L.lbm_get_macros.argtypes = [vp, dp, dp]
+    L.lbm_set_macros.argtypes = [vp, dp, dp]
+    L.lbm_step.argtypes = [vp, C.c_uint64, C.c_uint64]
+    L.lbm_sync.argtypes = [vp]
+    L.lbm_last_step_ms.argtypes = [vp, dp]
+    L.lbm_particles_set.argtypes = [vp, C.POINTER(abi.Particle), C.c_uint32,
+                                    C.POINTER(abi.ParticleParams)]
+    L.lbm_particles_get.argtypes = [vp, C.POINTER(abi.Particle), C.c_uint32]
+    L.lbm_spheroid_mass.argtypes = [C.c_int32, C.c_int32, C.c_double, C.c_double, dp, dp]
+    L.lbm_psm_couple.argtypes = [vp, C.c_uint64]
+    L.lbm_particle_reduce.argtypes = [vp, dp, dp]
+    L.lbm_dem_contact.argtypes = [vp]
+    L.lbm_particle_integrate.argtypes = [vp]
+    L.lbm_cell_list_overflow.argtypes = [vp, C.POINTER(C.c_uint64)]
+    for name in EXPORTS:
+        f = getattr(L, name)
+        if name not in ("lbm_last_error", "lbm_version", "lbm_device_count"):
+            f.restype = st
+    if path is None:
+        _lib = L
+    return L
+
+
+def check(status):
+    if status != abi.OK:
+        msg = load_library().lbm_last_error()
+        raise LbmError(status, msg.decode() if msg else "")
+
+
+def __getattr__(name):
+    # Lattice and friends import the library lazily so that ``abi`` stays
+    # importable on machines without the built library.
+    if name in ("Lattice", "components"):
+        from . import lattice as _lat
+        if name == "Lattice":
+            return _lat.Lattice
+        from . import components
+        return components
+    raise AttributeError(name)
