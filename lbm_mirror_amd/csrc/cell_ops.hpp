@@ -1,0 +1,261 @@
+// Per-cell operators of the fused stream+collide kernel.
+//
+// Each function works on one cell's populations held in registers.  The
+// arithmetic keeps the reference's operation order so that FP64 and FP32
+// results are bitwise reproducible (every `data<T>` operation rounds to T,
+// modules/core/c++/abstract/data.hpp:170-188); the library is built with
+// -ffp-contract=off.  Products with a lattice velocity component c ∈ {−1,0,1}
+// are written as add/subtract/skip, which is exact in IEEE arithmetic.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "lattice.hpp"
+#include "../../include/lbm_mirror_amd/lbm.h"
+
+namespace lbmx {
+
+// Device copy of one flag's operator (lbm_op_desc); T-valued parameters are
+// already rounded to T on the host and carried as double.
+struct OpDev {
+  int32_t bc, coll, east;
+  uint32_t flags;
+  double omega;  // T(T(1)/T(tau)), collision.hpp:25-38
+  double rho_start, rho_slope, rho_final;
+  uint64_t ramp_steps;
+  double vec[3];
+  double rho_eq;
+  double disc[5];
+};
+
+// compute_rho_u (macroscopic.hpp:10-35)
+template <class L, class T>
+__device__ __forceinline__ void rho_u(const T (&f)[L::Q], T& rho, T (&u)[3]) {
+  rho = T(0);
+  u[0] = u[1] = u[2] = T(0);
+#pragma unroll
+  for (int j = 0; j < L::Q; ++j) {
+    rho = rho + f[j];
+#pragma unroll
+    for (int d = 0; d < L::D; ++d) {
+      const int c = cdir<L>(j, d);
+      if (c == 1) u[d] = u[d] + f[j];
+      if (c == -1) u[d] = u[d] - f[j];
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < L::D; ++d) u[d] = u[d] / rho;
+}
+
+// equilibrium (equilibrium.hpp:7-48).  WB: the bracket is evaluated in
+// double and narrowed once, as the reference does for FP32 (its literals are
+// double); for T = double WB changes nothing.
+template <class L, class T, bool WB>
+__device__ __forceinline__ T eq_dir(int i, T rho, const T (&u)[3], T vv) {
+  T vc = T(0);
+#pragma unroll
+  for (int d = 0; d < L::D; ++d) {
+    const int c = cdir<L>(i, d);
+    if (c == 1) vc = vc + u[d];
+    if (c == -1) vc = vc - u[d];
+  }
+  const T a = vc * T(3.0);
+  const T wr = T(L::w[i]) * rho;
+  const T aa = a * a;
+  if (WB) {
+    const double br = ((1.0 + double(a)) - 1.5 * double(vv)) + double(aa) * 0.5;
+    return T(double(wr) * br);
+  } else {
+    return wr * (((T(1) + a) - T(1.5) * vv) + aa * T(0.5));
+  }
+}
+
+template <class L, class T>
+__device__ __forceinline__ T vel_sq(const T (&u)[3]) {
+  T vv = T(0);
+#pragma unroll
+  for (int d = 0; d < L::D; ++d) vv = vv + u[d] * u[d];
+  return vv;
+}
+
+template <class L, class T, bool WB>
+__device__ __forceinline__ void equilibrium(T rho, const T (&u)[3], T (&eq)[L::Q]) {
+  const T vv = vel_sq<L, T>(u);
+#pragma unroll
+  for (int i = 0; i < L::Q; ++i) eq[i] = eq_dir<L, T, WB>(i, rho, u, vv);
+}
+
+// BGK relaxation (collision.hpp:91-95)
+template <class L, class T, bool WB>
+__device__ __forceinline__ void bgk(T (&f)[L::Q], T rho, const T (&u)[3], T om) {
+  const T vv = vel_sq<L, T>(u);
+#pragma unroll
+  for (int i = 0; i < L::Q; ++i) {
+    const T e = eq_dir<L, T, WB>(i, rho, u, vv);
+    f[i] = f[i] + om * (e - f[i]);
+  }
+}
+
+// BGK with Guo source (collision.hpp:280-315), F = cell force + global force
+template <class L, class T, bool WB>
+__device__ __forceinline__ void bgk_guo(T (&f)[L::Q], T rho, const T (&u)[3], T om,
+                                        const T (&F)[3]) {
+  const T vv = vel_sq<L, T>(u);
+  const T relax = T(1) - T(0.5f) * om;
+  const T ics2 = T(3.0);
+#pragma unroll
+  for (int i = 0; i < L::Q; ++i) {
+    const T e = eq_dir<L, T, WB>(i, rho, u, vv);
+    T cu = T(0);
+#pragma unroll
+    for (int d = 0; d < L::D; ++d) {
+      const int c = cdir<L>(i, d);
+      if (c == 1) cu = cu + u[d];
+      if (c == -1) cu = cu - u[d];
+    }
+    const T s2 = (cu * ics2) * ics2;
+    T proj = T(0);
+#pragma unroll
+    for (int d = 0; d < L::D; ++d) {
+      const T cd = T(cdir<L>(i, d));
+      const T t1 = (cd - u[d]) * ics2;
+      const T t2 = cd * s2;
+      proj = proj + (t1 + t2) * F[d];
+    }
+    const T Fi = T(L::w[i]) * proj;
+    f[i] = (f[i] + om * (e - f[i])) + Fi * relax;
+  }
+}
+
+// PSM solid-fraction collision (psm.hpp:85-88)
+template <class L, class T, bool WB>
+__device__ __forceinline__ void psm(T (&f)[L::Q], T rho, const T (&u)[3], T om, T B) {
+  const T vv = vel_sq<L, T>(u);
+  const T flip = T(1.0) - B;
+  T cpy[L::Q];
+#pragma unroll
+  for (int i = 0; i < L::Q; ++i) cpy[i] = f[i];
+#pragma unroll
+  for (int i = 0; i < L::Q; ++i) {
+    const T e = eq_dir<L, T, WB>(i, rho, u, vv);
+    const int o = L::opp[i];
+    f[i] = (cpy[i] + (om * (e - cpy[i])) * B) + (cpy[o] - cpy[i]) * flip;
+  }
+}
+
+// Σ_i (f_i − f_opp(i)) c_i (momentum_gather.hpp:35-44, component/psm.hpp:64-73)
+template <class L, class T>
+__device__ __forceinline__ void opp_momentum(const T (&f)[L::Q], T (&m)[3]) {
+  m[0] = m[1] = m[2] = T(0);
+#pragma unroll
+  for (int i = 0; i < L::Q; ++i) {
+    const T diff = f[i] - f[L::opp[i]];
+#pragma unroll
+    for (int d = 0; d < L::D; ++d) {
+      const int c = cdir<L>(i, d);
+      if (c == 1) m[d] = m[d] + diff;
+      if (c == -1) m[d] = m[d] - diff;
+    }
+  }
+}
+
+// ZouHePressureX<Dir>, D2Q9 (boundary.hpp:186-226)
+template <class T>
+__device__ __forceinline__ void zouhe_p_d2q9(T* f, bool dir, T rs) {
+  T rvx;
+  if (dir)
+    rvx = rs - (((f[0] + f[3]) + f[4]) + ((f[1] + f[5]) + f[7]) * T(2u));
+  else
+    rvx = (((f[0] + f[3]) + f[4]) + ((f[2] + f[6]) + f[8]) * T(2u)) - rs;
+  const T c23 = T(2.0 / 3.0), c16 = T(1.0 / 6.0), h = T(0.5);
+  if (dir) {
+    f[2] = f[1] + c23 * rvx;
+    f[6] = (f[5] + c16 * rvx) + h * (f[3] - f[4]);
+    f[8] = (f[7] + c16 * rvx) + h * (f[4] - f[3]);
+  } else {
+    f[1] = f[2] - c23 * rvx;
+    f[5] = (f[6] - c16 * rvx) + h * (f[4] - f[3]);
+    f[7] = (f[8] - c16 * rvx) + h * (f[3] - f[4]);
+  }
+}
+
+// ZouHePressureX<East>, D3Q27 (boundary.hpp:256-754); the body is written for
+// the x = max face and the flag is flipped once (AtXMax = !East, :248).
+template <class T>
+__device__ __forceinline__ void zouhe_p_d3q27(T* f, bool east, T rs) {
+  const bool at_max = !east;
+  const T two = T(2.0);
+  const T S0 = ((((((((f[0] + f[3]) + f[6]) + f[9]) + f[12]) + f[15]) + f[18]) + f[21]) + f[24]);
+  const T y0 = (((((-f[3]) + f[6]) - f[12]) + f[15]) - f[21]) + f[24];
+  const T z0 = (((((-f[9]) - f[12]) - f[15]) + f[18]) + f[21]) + f[24];
+  T ux, uy, uz;
+  if (at_max) {
+    const T k = ((((((((f[2] + f[5]) + f[8]) + f[11]) + f[14]) + f[17]) + f[20]) + f[23]) + f[26]);
+    ux = (S0 + two * k) - rs;
+    uy = y0 + two * (((((-f[5]) + f[8]) - f[14]) + f[17]) - f[23] + f[26]);
+    uz = z0 + two * (((((-f[11]) - f[14]) - f[17]) + f[20]) + f[23] + f[26]);
+  } else {
+    const T k = ((((((((f[1] + f[4]) + f[7]) + f[10]) + f[13]) + f[16]) + f[19]) + f[22]) + f[25]);
+    ux = rs - (S0 + two * k);
+    uy = y0 + two * (((((-f[4]) + f[7]) - f[13]) + f[16]) - f[22] + f[25]);
+    uz = z0 + two * (((((-f[10]) - f[13]) - f[16]) + f[19]) + f[22] + f[25]);
+  }
+  const T c49 = T(4.0 / 9.0), c19 = T(1.0 / 9.0), c136 = T(1.0 / 36.0);
+  if (at_max) {
+    f[1] = f[2] - c49 * ux;
+    f[4] = f[5] - c19 * (ux + uy);
+    f[7] = f[8] + c19 * ((-ux) + uy);
+    f[10] = f[11] - c19 * (ux + uz);
+    f[13] = f[14] - c136 * ((ux + uy) + uz);
+    f[16] = f[17] + c136 * (((-ux) + uy) + uz);
+    f[19] = f[20] + c19 * ((-ux) + uz);
+    f[22] = f[23] + c136 * (((-ux) - uy) + uz);
+    f[25] = f[26] + c136 * (((-ux) + uy) + uz);
+  } else {
+    f[2] = f[1] + c49 * ux;
+    f[5] = f[4] + c19 * (ux - uy);
+    f[8] = f[7] + c19 * (ux + uy);
+    f[11] = f[10] + c19 * (ux - uz);
+    f[14] = f[13] + c136 * ((ux - uy) - uz);
+    f[17] = f[16] + c136 * ((ux + uy) - uz);
+    f[20] = f[19] + c19 * (ux + uz);
+    f[23] = f[22] + c136 * ((ux - uy) + uz);
+    f[26] = f[25] + c136 * ((ux + uy) + uz);
+  }
+}
+
+// ZouHeVelocityX<East>, D2Q9 index-hard-coded (boundary.hpp:769-803)
+template <class T>
+__device__ __forceinline__ void zouhe_u_d2q9(T* f, bool east, T u0, T u1) {
+  const T ds = east ? ((f[1] + f[5]) + f[7]) : ((f[2] + f[6]) + f[8]);
+  const T r = (((f[0] + f[4]) + f[3]) + ds * T(2)) / (u0 + T(1.0));
+  const T c23 = T(2.0 / 3.0), c16 = T(1.0 / 6.0), h = T(0.5f);
+  if (east) {
+    f[2] = f[1] + (c23 * r) * u0;
+    f[6] = (f[5] + (c16 * r) * u0) + (((f[3] - f[4]) + r * u1) * h);
+    f[8] = (f[7] + (c16 * r) * u0) + (((f[4] - f[3]) + r * u1) * h);
+  } else {
+    f[1] = f[2] + (c23 * r) * u0;
+    f[5] = (f[6] + (c16 * r) * u0) + (((f[4] - f[3]) + r * u1) * h);
+    f[7] = (f[8] + (c16 * r) * u0) + (((f[3] - f[4]) + r * u1) * h);
+  }
+}
+
+// particle_porosity<..., ParticleSpheroid>::calculate (porosity.hpp:56-89).
+// For T = float the ramp argument is formed in double (π is a double) and
+// narrowed before sinf, as the reference's expression does.
+template <class T, int D>
+__device__ __forceinline__ T disc_porosity(const T* rel, T rad, T eps) {
+  T s2 = T(0);
+#pragma unroll
+  for (int d = 0; d < D; ++d) s2 = s2 + rel[d] * rel[d];
+  const T s = sqrt(s2);
+  const T eps_h = eps / T(2);
+  const T lo = rad - eps_h, hi = rad + eps_h;
+  if (s <= lo) return T(0);
+  if (s >= hi) return T(1);
+  const T inner = T((3.14159265358979323846 / double(T(2) * eps)) * double(s - lo));
+  const T si = sin(inner);
+  return si * si;
+}
+
+}  // namespace lbmx

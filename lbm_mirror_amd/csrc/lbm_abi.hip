@@ -1,0 +1,754 @@
+// C ABI of the engine (include/lbm_mirror_amd/lbm.h).
+//
+// One lbm_ctx owns one lattice on one GPU: its population buffer (AA layout,
+// single copy), flags, operator table, macro fields and particle state, and
+// one HIP stream on which every launch of the context is ordered.  No C++
+// exception crosses the boundary; every entry point returns an lbm_status and
+// leaves a message for lbm_last_error().
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "fields.hpp"
+#include "kernels.hpp"
+
+using namespace lbmx;
+
+namespace {
+
+thread_local std::string g_err;
+
+lbm_status fail(lbm_status s, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return s;
+}
+
+#define HIP_TRY(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(e_ == hipErrorOutOfMemory ? LBM_ERR_OUT_OF_MEMORY : LBM_ERR_CRITICAL, \
+                  "%s: %s", #expr, hipGetErrorString(e_));                             \
+  } while (0)
+
+int dims_of(int lat) { return lat == LBM_D2Q9 ? 2 : 3; }
+int q_of(int lat) { return lat == LBM_D2Q9 ? 9 : (lat == LBM_D3Q19 ? 19 : 27); }
+size_t storage_bytes(int st) { return st == LBM_F64 ? 8 : (st == LBM_F32 ? 4 : 2); }
+bool compute_double(int st) { return st == LBM_F64; }
+
+// Rounds a double parameter to the storage's compute type T, as the
+// reference's component constructors do (native_data_type<T>).
+double to_T(int st, double v) { return compute_double(st) ? v : double(float(v)); }
+
+const int kMaxParticles = 4096;
+
+}  // namespace
+
+struct lbm_ctx {
+  int lattice = 0, storage = 0, D = 2, Q = 9, device = 0;
+  uint64_t n[3] = {1, 1, 1};
+  uint32_t N = 0, Np = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool timed = false;
+  void* f = nullptr;
+  uint8_t* flags = nullptr;
+  OpDev* ops = nullptr;
+  OpDev ops_h[kMaxOps];
+  bool has_op[kMaxOps] = {};
+  void* rho = nullptr;
+  void* u = nullptr;
+  void* force = nullptr;
+  void* por = nullptr;
+  void* bcvel = nullptr;
+  uint32_t periodic = 0;
+  int phase = 0;
+  bool flags_set = false;
+  LaunchShape shape;
+  // particles
+  uint32_t np = 0;
+  void* parts = nullptr;
+  double* f64 = nullptr;
+  double* t64 = nullptr;
+  double* partial = nullptr;
+  uint32_t bpp = 1;
+  lbm_particle_params pp{};
+  uint64_t* cl_count = nullptr;
+  uint64_t* cl_slot = nullptr;
+  uint64_t* cl_overflow = nullptr;
+  uint32_t* owner = nullptr;
+  double bin = 1.0;
+  uint64_t grid[3] = {1, 1, 1};
+  uint64_t bins = 1;
+  size_t tsz() const { return compute_double(storage) ? 8 : 4; }
+};
+
+namespace {
+
+hipError_t alloc_zero(void** p, size_t bytes, hipStream_t s) {
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) return e;
+  return hipMemsetAsync(*p, 0, bytes, s);
+}
+
+lbm_status ensure_field(lbm_ctx* c, void** field, size_t count, double fill) {
+  if (*field) return LBM_OK;
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(alloc_zero(field, count * c->tsz(), c->stream));
+  if (fill != 0.0) {
+    std::vector<double> h(count, fill);
+    double* tmp = nullptr;
+    HIP_TRY(hipMalloc(&tmp, count * sizeof(double)));
+    HIP_TRY(hipMemcpyAsync(tmp, h.data(), count * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(field_from_double(compute_double(c->storage), *field, tmp, count, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipFree(tmp));
+  }
+  return LBM_OK;
+}
+
+Geometry geom(const lbm_ctx* c) {
+  Geometry g;
+  g.nx = int32_t(c->n[0]);
+  g.ny = int32_t(c->n[1]);
+  g.nz = int32_t(c->n[2]);
+  g.N = c->N;
+  g.Np = c->Np;
+  g.periodic = c->periodic;
+  return g;
+}
+
+lbm_status put_field(lbm_ctx* c, void* field, const double* host, size_t count) {
+  double* tmp = nullptr;
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipMalloc(&tmp, count * sizeof(double)));
+  HIP_TRY(hipMemcpyAsync(tmp, host, count * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(field_from_double(compute_double(c->storage), field, tmp, count, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipFree(tmp));
+  return LBM_OK;
+}
+
+lbm_status get_field(lbm_ctx* c, const void* field, double* host, size_t count) {
+  double* tmp = nullptr;
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipMalloc(&tmp, count * sizeof(double)));
+  HIP_TRY(field_to_double(compute_double(c->storage), field, tmp, count, c->stream));
+  HIP_TRY(hipMemcpyAsync(host, tmp, count * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipFree(tmp));
+  return LBM_OK;
+}
+
+ParticleArgs particle_args(lbm_ctx* c) {
+  ParticleArgs a{};
+  a.f = c->f;
+  a.por = c->por;
+  a.force = c->force;
+  a.nx = int32_t(c->n[0]);
+  a.ny = int32_t(c->n[1]);
+  a.nz = int32_t(c->n[2]);
+  a.N = c->N;
+  a.Np = c->Np;
+  a.periodic = c->periodic;
+  a.phase = c->phase;
+  a.parts = c->parts;
+  a.n = c->np;
+  a.f64 = c->f64;
+  a.t64 = c->t64;
+  a.partial = c->partial;
+  a.blocks_per_particle = c->bpp;
+  a.eps = to_T(c->storage, c->pp.eps);
+  a.sub_steps = c->pp.sub_steps;
+  a.porosity_mode = c->pp.porosity_mode;
+  a.psm_torque = c->pp.psm_torque;
+  a.k_n = c->pp.k_n;
+  a.zeta_n = c->pp.zeta_n;
+  a.gamma_t = c->pp.gamma_t;
+  a.mu = c->pp.mu;
+  a.walls = c->pp.walls;
+  for (int d = 0; d < 3; ++d) a.g[d] = to_T(c->storage, c->pp.body_acceleration[d]);
+  a.cl_count = c->cl_count;
+  a.cl_slot = c->cl_slot;
+  a.cl_overflow = c->cl_overflow;
+  a.owner = c->owner;
+  a.bin = c->bin;
+  for (int d = 0; d < 3; ++d) a.grid[d] = c->grid[d];
+  a.cap = uint32_t(c->pp.cell_capacity);
+  return a;
+}
+
+lbm_status check_ready(lbm_ctx* c) {
+  if (!c->flags_set) return fail(LBM_ERR_INVALID_STATE, "lbm_set_flags has not been called");
+  for (int k = 0; k < kMaxOps; ++k) {
+    if (!c->has_op[k]) continue;
+    const OpDev& o = c->ops_h[k];
+    if (o.coll == LBM_COLL_PSM && !c->por) return fail(LBM_ERR_INVALID_STATE, "PSM without porosity field");
+    if (o.bc == LBM_BC_ZOUHE_VELOCITY_X && !c->bcvel)
+      return fail(LBM_ERR_INVALID_STATE, "Zou/He velocity without lbm_set_boundary_velocity");
+  }
+  return LBM_OK;
+}
+
+lbm_status particle_pass(lbm_ctx* c, uint64_t t) {
+  (void)t;
+  ParticleArgs a = particle_args(c);
+  HIP_TRY(launch_psm_couple(c->lattice, c->storage, a, c->stream));
+  HIP_TRY(launch_dem_contact(c->D, c->storage, a, c->stream));
+  HIP_TRY(launch_particle_integrate(c->D, c->storage, a, c->stream));
+  return LBM_OK;
+}
+
+lbm_status fluid_pass(lbm_ctx* c, uint64_t t, bool write_macros) {
+  StepArgs a;
+  a.f = c->f;
+  a.flags = c->flags;
+  a.ops = c->ops;
+  a.rho = c->rho;
+  a.u = c->u;
+  a.force = c->force;
+  a.por = c->por;
+  a.bcvel = c->bcvel;
+  a.nx = int32_t(c->n[0]);
+  a.ny = int32_t(c->n[1]);
+  a.nz = int32_t(c->n[2]);
+  a.N = c->N;
+  a.Np = c->Np;
+  a.periodic = c->periodic;
+  a.t = t;
+  a.write_macros = write_macros ? 1 : 0;
+  HIP_TRY(launch_aa_step(c->lattice, c->storage, c->phase == 1, a, c->shape, c->stream));
+  c->phase ^= 1;
+  return LBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* lbm_last_error(void) { return g_err.c_str(); }
+const char* lbm_version(void) { return "lbm_mirror_amd 0.1 (gfx950)"; }
+
+int lbm_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+lbm_status lbm_ctx_create(lbm_ctx** out, int32_t lattice, int32_t storage,
+                          const uint64_t extent[3], int32_t device) {
+  if (!out || !extent) return fail(LBM_ERR_INVALID_ARG, "null argument");
+  *out = nullptr;
+  if (lattice < LBM_D2Q9 || lattice > LBM_D3Q27) return fail(LBM_ERR_INVALID_ARG, "unknown lattice %d", lattice);
+  if (storage < LBM_F64 || storage > LBM_BF16S) return fail(LBM_ERR_INVALID_ARG, "unknown storage %d", storage);
+  const int D = dims_of(lattice);
+  uint64_t n[3] = {extent[0], extent[1], D == 3 ? extent[2] : 1};
+  uint64_t N = 1;
+  for (int d = 0; d < 3; ++d) {
+    if (n[d] == 0) return fail(LBM_ERR_INVALID_ARG, "empty extent on axis %d", d);
+    N *= n[d];
+  }
+  const int Q = q_of(lattice);
+  const uint64_t Np = (N + 255) / 256 * 256;
+  if (N >= (1ull << 31) || n[0] >= (1u << 30) || n[1] >= (1u << 30) || n[2] >= 65536)
+    return fail(LBM_ERR_NOT_SUPPORTED, "lattice too large for 32-bit cell indices");
+  int ndev = lbm_device_count();
+  if (ndev < 1) return fail(LBM_ERR_NOT_FOUND, "no GPU visible");
+  if (device < 0 || device >= ndev) return fail(LBM_ERR_INVALID_ARG, "device %d of %d", device, ndev);
+  lbm_ctx* c = new lbm_ctx();
+  c->lattice = lattice;
+  c->storage = storage;
+  c->D = D;
+  c->Q = Q;
+  c->device = device;
+  for (int d = 0; d < 3; ++d) c->n[d] = n[d];
+  c->N = uint32_t(N);
+  c->Np = uint32_t(Np);
+  std::memset(c->ops_h, 0, sizeof(c->ops_h));
+  auto bail = [&](lbm_status s) {
+    lbm_ctx_destroy(c);
+    return s;
+  };
+  if (hipSetDevice(device) != hipSuccess) return bail(fail(LBM_ERR_CRITICAL, "hipSetDevice"));
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(LBM_ERR_CRITICAL, "hipStreamCreate"));
+  if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
+    return bail(fail(LBM_ERR_CRITICAL, "hipEventCreate"));
+  hipError_t e = alloc_zero(&c->f, size_t(Q) * Np * storage_bytes(storage), c->stream);
+  if (e == hipSuccess) e = alloc_zero((void**)&c->flags, N, c->stream);
+  if (e == hipSuccess) e = alloc_zero((void**)&c->ops, sizeof(OpDev) * kMaxOps, c->stream);
+  if (e == hipSuccess) e = alloc_zero(&c->rho, N * c->tsz(), c->stream);
+  if (e == hipSuccess) e = alloc_zero(&c->u, size_t(D) * N * c->tsz(), c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess)
+    return bail(fail(e == hipErrorOutOfMemory ? LBM_ERR_OUT_OF_MEMORY : LBM_ERR_CRITICAL,
+                     "allocation: %s", hipGetErrorString(e)));
+  // block shape: one row segment of up to 256 cells, rows stacked to fill 256 threads
+  int bx = 64;
+  while (bx < 256 && uint64_t(bx) < n[0]) bx *= 2;
+  c->shape.bx = bx;
+  c->shape.by = 256 / bx;
+  *out = c;
+  return LBM_OK;
+}
+
+lbm_status lbm_ctx_destroy(lbm_ctx* c) {
+  if (!c) return LBM_OK;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  void* ptrs[] = {c->f, c->flags, c->ops, c->rho, c->u, c->force, c->por, c->bcvel,
+                  c->parts, c->f64, c->t64, c->partial, c->cl_count, c->cl_slot,
+                  c->cl_overflow, c->owner};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+  if (c->ev0) hipEventDestroy(c->ev0);
+  if (c->ev1) hipEventDestroy(c->ev1);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return LBM_OK;
+}
+
+lbm_status lbm_set_flags(lbm_ctx* c, const uint8_t* flags) {
+  if (!c || !flags) return fail(LBM_ERR_INVALID_ARG, "null argument");
+  for (uint32_t i = 0; i < c->N; ++i)
+    if (flags[i] == 0)
+      return fail(LBM_ERR_NOT_SUPPORTED,
+                  "cell %u has flag 0 (ghost) inside the lattice; the AA single-buffer layout "
+                  "keeps no stale second buffer for it",
+                  i);
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipMemcpyAsync(c->flags, flags, c->N, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->flags_set = true;
+  return LBM_OK;
+}
+
+lbm_status lbm_set_periodic(lbm_ctx* c, uint32_t mask) {
+  if (!c) return fail(LBM_ERR_INVALID_ARG, "null ctx");
+  if (mask > 7) return fail(LBM_ERR_INVALID_ARG, "axis mask %u", mask);
+  if (c->D == 2) mask &= 3u;
+  if (c->phase != 0) {
+    // the swapped layout's ghost handling depends on the wrap; change it only
+    // in the natural layout
+    return fail(LBM_ERR_INVALID_STATE, "set periodicity after an even number of steps");
+  }
+  c->periodic = mask;
+  return LBM_OK;
+}
+
+lbm_status lbm_set_operator(lbm_ctx* c, uint8_t flag, const lbm_op_desc* op) {
+  if (!c || !op) return fail(LBM_ERR_INVALID_ARG, "null argument");
+  if (flag == 0) return fail(LBM_ERR_INVALID_ARG, "flag 0 is the ghost flag and carries no operator");
+  if (flag >= kMaxOps) return fail(LBM_ERR_NOT_SUPPORTED, "operators on flags >= %d", kMaxOps);
+  if (op->boundary < LBM_BC_NONE || op->boundary > LBM_BC_EQUILIBRIUM)
+    return fail(LBM_ERR_INVALID_ARG, "unknown boundary kind %d", op->boundary);
+  if (op->collision < LBM_COLL_NONE || op->collision > LBM_COLL_PSM)
+    return fail(LBM_ERR_INVALID_ARG, "unknown collision kind %d", op->collision);
+  if (op->boundary == LBM_BC_ZOUHE_PRESSURE_X && c->lattice == LBM_D3Q19)
+    return fail(LBM_ERR_NOT_SUPPORTED, "the reference has Zou/He pressure for D2Q9 and D3Q27 only");
+  if ((op->boundary == LBM_BC_ZOUHE_VELOCITY_X || op->boundary == LBM_BC_ANTI_BOUNCE_BACK) &&
+      c->lattice != LBM_D2Q9)
+    return fail(LBM_ERR_NOT_SUPPORTED, "this boundary is D2Q9-index-hard-coded in the reference");
+  if (op->collision != LBM_COLL_NONE && !(op->tau > 0.0))
+    return fail(LBM_ERR_INVALID_ARG, "tau must be positive");
+  const int st = c->storage;
+  OpDev o{};
+  o.bc = op->boundary;
+  o.coll = op->collision;
+  o.east = op->east;
+  o.flags = op->flags;
+  o.omega = compute_double(st) ? 1.0 / op->tau : double(1.0f / float(op->tau));
+  o.rho_start = op->rho_start;
+  o.rho_slope = op->rho_slope;
+  o.rho_final = op->rho_final;
+  o.ramp_steps = op->ramp_steps;
+  for (int d = 0; d < 3; ++d) o.vec[d] = to_T(st, op->vec[d]);
+  o.rho_eq = to_T(st, op->rho_eq);
+  for (int k = 0; k < 5; ++k) o.disc[k] = to_T(st, op->disc[k]);
+  if (o.coll == LBM_COLL_PSM) {
+    lbm_status s = ensure_field(c, &c->por, c->N, 1.0);
+    if (s) return s;
+  }
+  if ((o.coll == LBM_COLL_BGK_GUO && (o.flags & LBM_OPF_GUO_CELL_FORCE)) ||
+      (o.coll == LBM_COLL_PSM && (o.flags & (LBM_OPF_PSM_CELL_FORCE | LBM_OPF_PSM_FORCE_GATHER)))) {
+    lbm_status s = ensure_field(c, &c->force, size_t(c->D) * c->N, 0.0);
+    if (s) return s;
+  }
+  c->ops_h[flag] = o;
+  c->has_op[flag] = true;
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipMemcpyAsync(c->ops + flag, &o, sizeof(OpDev), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return LBM_OK;
+}
+
+lbm_status lbm_set_boundary_velocity(lbm_ctx* c, const double* u) {
+  if (!c || !u) return fail(LBM_ERR_INVALID_ARG, "null argument");
+  lbm_status s = ensure_field(c, &c->bcvel, size_t(c->D) * c->N, 0.0);
+  if (s) return s;
+  return put_field(c, c->bcvel, u, size_t(c->D) * c->N);
+}
+
+lbm_status lbm_set_force_field(lbm_ctx* c, const double* f) {
+  if (!c || !f) return fail(LBM_ERR_INVALID_ARG, "null argument");
+  lbm_status s = ensure_field(c, &c->force, size_t(c->D) * c->N, 0.0);
+  if (s) return s;
+  return put_field(c, c->force, f, size_t(c->D) * c->N);
+}
+
+lbm_status lbm_get_force_field(lbm_ctx* c, double* f) {
+  if (!c || !f) return fail(LBM_ERR_INVALID_ARG, "null argument");
+  if (!c->force) {
+    std::fill(f, f + size_t(c->D) * c->N, 0.0);
+    return LBM_OK;
+  }
+  return get_field(c, c->force, f, size_t(c->D) * c->N);
+}
+
+lbm_status lbm_set_porosity(lbm_ctx* c, const double* p) {
+  if (!c || !p) return fail(LBM_ERR_INVALID_ARG, "null argument");
+  lbm_status s = ensure_field(c, &c->por, c->N, 1.0);
+  if (s) return s;
+  return put_field(c, c->por, p, c->N);
+}
+
+lbm_status lbm_get_porosity(lbm_ctx* c, double* p) {
+  if (!c || !p) return fail(LBM_ERR_INVALID_ARG, "null argument");
+  if (!c->por) {
+    std::fill(p, p + c->N, 1.0);
+    return LBM_OK;
+  }
+  return get_field(c, c->por, p, c->N);
+}
+
+lbm_status lbm_set_populations(lbm_ctx* c, const double* f) {
+  if (!c || !f) return fail(LBM_ERR_INVALID_ARG, "null argument");
+  const size_t cnt = size_t(c->Q) * c->N;
+  double* tmp = nullptr;
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipMalloc(&tmp, cnt * sizeof(double)));
+  HIP_TRY(hipMemcpyAsync(tmp, f, cnt * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(pack_populations(c->lattice, c->storage, c->f, tmp, geom(c), c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipFree(tmp));
+  c->phase = 0;
+  return LBM_OK;
+}
+
+lbm_status lbm_get_populations(lbm_ctx* c, double* f) {
+  if (!c || !f) return fail(LBM_ERR_INVALID_ARG, "null argument");
+  const size_t cnt = size_t(c->Q) * c->N;
+  double* tmp = nullptr;
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipMalloc(&tmp, cnt * sizeof(double)));
+  HIP_TRY(unpack_populations(c->lattice, c->storage, c->f, tmp, geom(c), c->phase, c->stream));
+  HIP_TRY(hipMemcpyAsync(f, tmp, cnt * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipFree(tmp));
+  return LBM_OK;
+}
+
+lbm_status lbm_init_equilibrium(lbm_ctx* c, double rho, const double u[3]) {
+  if (!c || !u) return fail(LBM_ERR_INVALID_ARG, "null argument");
+  // eq(ρ, u) in T with the reference's arithmetic, evaluated on the host
+  double eq[27];
+  const int st = c->storage;
+  auto fill = [&](auto L, auto zero) {
+    using LL = decltype(L);
+    using T = decltype(zero);
+    T uu[3] = {T(u[0]), T(u[1]), T(u[2])};
+    T r = T(rho);
+    T vv = T(0);
+    for (int d = 0; d < LL::D; ++d) vv = T(vv + T(uu[d] * uu[d]));
+    for (int i = 0; i < LL::Q; ++i) {
+      T vc = T(0);
+      for (int d = 0; d < LL::D; ++d) vc = T(vc + T(uu[d] * T(cdir<LL>(i, d))));
+      T a = T(vc * T(3.0));
+      T wr = T(T(LL::w[i]) * r);
+      T aa = T(a * a);
+      eq[i] = double(T(double(wr) * (((1.0 + double(a)) - 1.5 * double(vv)) + double(aa) * 0.5)));
+    }
+  };
+  const bool dbl = compute_double(st);
+  if (c->lattice == LBM_D2Q9) dbl ? fill(D2Q9{}, 0.0) : fill(D2Q9{}, 0.0f);
+  if (c->lattice == LBM_D3Q19) dbl ? fill(D3Q19{}, 0.0) : fill(D3Q19{}, 0.0f);
+  if (c->lattice == LBM_D3Q27) dbl ? fill(D3Q27{}, 0.0) : fill(D3Q27{}, 0.0f);
+  std::vector<double> h(size_t(c->Q) * c->N);
+  for (int i = 0; i < c->Q; ++i) std::fill(h.begin() + size_t(i) * c->N, h.begin() + size_t(i + 1) * c->N, eq[i]);
+  lbm_status s = lbm_set_populations(c, h.data());
+  if (s) return s;
+  std::vector<double> r(c->N, to_T(st, rho)), uv(size_t(c->D) * c->N);
+  for (int d = 0; d < c->D; ++d)
+    std::fill(uv.begin() + size_t(d) * c->N, uv.begin() + size_t(d + 1) * c->N, to_T(st, u[d]));
+  return lbm_set_macros(c, r.data(), uv.data());
+}
+
+lbm_status lbm_get_macros(lbm_ctx* c, double* rho, double* u) {
+  if (!c) return fail(LBM_ERR_INVALID_ARG, "null ctx");
+  if (rho) {
+    lbm_status s = get_field(c, c->rho, rho, c->N);
+    if (s) return s;
+  }
+  if (u) return get_field(c, c->u, u, size_t(c->D) * c->N);
+  return LBM_OK;
+}
+
+lbm_status lbm_set_macros(lbm_ctx* c, const double* rho, const double* u) {
+  if (!c) return fail(LBM_ERR_INVALID_ARG, "null ctx");
+  if (rho) {
+    lbm_status s = put_field(c, c->rho, rho, c->N);
+    if (s) return s;
+  }
+  if (u) return put_field(c, c->u, u, size_t(c->D) * c->N);
+  return LBM_OK;
+}
+
+lbm_status lbm_step(lbm_ctx* c, uint64_t t0, uint64_t nsteps) {
+  if (!c) return fail(LBM_ERR_INVALID_ARG, "null ctx");
+  lbm_status s = check_ready(c);
+  if (s) return s;
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipEventRecord(c->ev0, c->stream));
+  for (uint64_t k = 0; k < nsteps; ++k) {
+    const uint64_t t = t0 + k;
+    if (c->np > 0) {
+      s = particle_pass(c, t);
+      if (s) return s;
+    }
+    s = fluid_pass(c, t, k + 1 == nsteps);
+    if (s) return s;
+  }
+  HIP_TRY(hipEventRecord(c->ev1, c->stream));
+  c->timed = true;
+  return LBM_OK;
+}
+
+lbm_status lbm_sync(lbm_ctx* c) {
+  if (!c) return fail(LBM_ERR_INVALID_ARG, "null ctx");
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return LBM_OK;
+}
+
+lbm_status lbm_last_step_ms(lbm_ctx* c, double* ms) {
+  if (!c || !ms) return fail(LBM_ERR_INVALID_ARG, "null argument");
+  if (!c->timed) return fail(LBM_ERR_INVALID_STATE, "no lbm_step call yet");
+  HIP_TRY(hipEventSynchronize(c->ev1));
+  float m = 0.f;
+  HIP_TRY(hipEventElapsedTime(&m, c->ev0, c->ev1));
+  *ms = m;
+  return LBM_OK;
+}
+
+// ---------------------------------------------------------------- particles
+lbm_status lbm_spheroid_mass(int32_t dims, int32_t storage, double r, double rho, double* mass,
+                             double* inertia) {
+  if (!mass || !inertia) return fail(LBM_ERR_INVALID_ARG, "null argument");
+  if (dims != 2 && dims != 3) return fail(LBM_ERR_INVALID_ARG, "dims %d", dims);
+  // create_spheroid_particle_group (particle/particle.hpp:79-101)
+  auto run = [&](auto zero) {
+    using T = decltype(zero);
+    constexpr T pi = T(3.14159265358979323846);
+    const T rr = T(r), de = T(rho);
+    if (dims == 2) {
+      auto m = T(T(T(pi * rr) * rr) * de);
+      *mass = m;
+      *inertia = T(0.5 * m * rr * rr);
+    } else {
+      auto m = (4.0 / 3.0) * pi * rr * rr * rr * de;
+      *mass = T(m);
+      *inertia = T(0.4 * m * rr * rr);
+    }
+  };
+  if (storage == LBM_F64) run(0.0);
+  else run(0.0f);
+  return LBM_OK;
+}
+
+lbm_status lbm_particles_set(lbm_ctx* c, const lbm_particle* p, uint32_t n,
+                             const lbm_particle_params* prm) {
+  if (!c || (!p && n) || !prm) return fail(LBM_ERR_INVALID_ARG, "null argument");
+  if (n > uint32_t(kMaxParticles)) return fail(LBM_ERR_NOT_SUPPORTED, "more than %d particles", kMaxParticles);
+  if (prm->porosity_mode != LBM_POROSITY_SCATTER && prm->porosity_mode != LBM_POROSITY_PRODUCT)
+    return fail(LBM_ERR_INVALID_ARG, "porosity mode %d", prm->porosity_mode);
+  if (prm->cell_capacity <= 0 && prm->porosity_mode == LBM_POROSITY_PRODUCT)
+    return fail(LBM_ERR_INVALID_ARG, "cell capacity must be positive");
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  const int st = c->storage;
+  const bool dbl = compute_double(st);
+  // (re)allocate
+  if (n != c->np) {
+    void* ptrs[] = {c->parts, c->f64, c->t64, c->partial};
+    for (void* q : ptrs)
+      if (q) HIP_TRY(hipFree(q));
+    c->parts = nullptr;
+    c->f64 = c->t64 = c->partial = nullptr;
+    c->np = 0;
+    if (n > 0) {
+      HIP_TRY(alloc_zero(&c->parts, size_t(n) * kPartFields * c->tsz(), c->stream));
+      HIP_TRY(alloc_zero((void**)&c->f64, size_t(n) * 3 * sizeof(double), c->stream));
+      HIP_TRY(alloc_zero((void**)&c->t64, size_t(n) * 3 * sizeof(double), c->stream));
+    }
+  }
+  c->pp = *prm;
+  if (c->pp.cell_capacity <= 0) c->pp.cell_capacity = 4;
+  // blocks per particle so that one block sweeps at most ~1024 AABB cells
+  double rmax = 0.0;
+  for (uint32_t k = 0; k < n; ++k) rmax = std::max(rmax, p[k].radius);
+  uint64_t box = 1;
+  for (int d = 0; d < c->D; ++d) box *= uint64_t(2.0 * rmax + 3.0);
+  uint32_t bpp = uint32_t(std::min<uint64_t>(64, std::max<uint64_t>(1, (box + 1023) / 1024)));
+  if (bpp != c->bpp || !c->partial) {
+    if (c->partial) HIP_TRY(hipFree(c->partial));
+    c->partial = nullptr;
+    c->bpp = bpp;
+    if (n > 0) HIP_TRY(alloc_zero((void**)&c->partial, size_t(n) * bpp * 6 * sizeof(double), c->stream));
+  }
+  if (n > 0) {
+    lbm_status s = ensure_field(c, &c->por, c->N, 1.0);
+    if (s) return s;
+    s = ensure_field(c, &c->force, size_t(c->D) * c->N, 0.0);
+    if (s) return s;
+    if (!c->owner) HIP_TRY(alloc_zero((void**)&c->owner, size_t(c->N) * 2 * sizeof(uint32_t), c->stream));
+  }
+  // cell list (make_cell_list, particle/cell_list.hpp:185-206) in T
+  {
+    double b = dbl ? prm->max_radius * 2 : double(float(prm->max_radius) * 2.0f);
+    if (!(b > 1)) b = 1;
+    c->bin = b;
+    uint64_t bins = 1;
+    for (int d = 0; d < 3; ++d) {
+      if (d < c->D) {
+        double e = double(c->n[d]);
+        double q = dbl ? e / b : double(float(e) / float(b));
+        c->grid[d] = uint64_t(q) + 1u;
+      } else {
+        c->grid[d] = 1;
+      }
+      bins *= c->grid[d];
+    }
+    if (c->cl_count) HIP_TRY(hipFree(c->cl_count));
+    if (c->cl_slot) HIP_TRY(hipFree(c->cl_slot));
+    if (!c->cl_overflow) HIP_TRY(alloc_zero((void**)&c->cl_overflow, sizeof(uint64_t), c->stream));
+    c->cl_count = c->cl_slot = nullptr;
+    c->bins = bins;
+    HIP_TRY(alloc_zero((void**)&c->cl_count, bins * sizeof(uint64_t), c->stream));
+    HIP_TRY(alloc_zero((void**)&c->cl_slot, bins * uint64_t(c->pp.cell_capacity) * sizeof(uint64_t), c->stream));
+  }
+  // records in T
+  if (n > 0) {
+    std::vector<double> rec(size_t(n) * kPartFields, 0.0);
+    for (uint32_t k = 0; k < n; ++k) {
+      double* r = rec.data() + size_t(k) * kPartFields;
+      for (int d = 0; d < 3; ++d) {
+        r[PF_POS + d] = to_T(st, p[k].position[d]);
+        r[PF_POS_OLD + d] = to_T(st, p[k].position_old[d]);
+        r[PF_ROT + d] = to_T(st, p[k].rotation[d]);
+        r[PF_ROT_OLD + d] = to_T(st, p[k].rotation_old[d]);
+        r[PF_ACC + d] = to_T(st, p[k].acceleration[d]);
+        r[PF_ANG_ACC + d] = to_T(st, p[k].angular_acceleration[d]);
+        r[PF_FORCE + d] = to_T(st, p[k].force[d]);
+        r[PF_TORQUE + d] = to_T(st, p[k].torque[d]);
+      }
+      r[PF_MASS] = to_T(st, p[k].mass);
+      r[PF_INERTIA] = to_T(st, p[k].inertia);
+      r[PF_DENSITY] = to_T(st, p[k].density);
+      r[PF_RADIUS] = to_T(st, p[k].radius);
+    }
+    c->np = n;
+    lbm_status s = put_field(c, c->parts, rec.data(), rec.size());
+    if (s) return s;
+  }
+  c->np = n;
+  return LBM_OK;
+}
+
+lbm_status lbm_particles_get(lbm_ctx* c, lbm_particle* p, uint32_t n) {
+  if (!c || (!p && n)) return fail(LBM_ERR_INVALID_ARG, "null argument");
+  if (n > c->np) return fail(LBM_ERR_INVALID_ARG, "asked for %u of %u particles", n, c->np);
+  if (n == 0) return LBM_OK;
+  std::vector<double> rec(size_t(c->np) * kPartFields);
+  lbm_status s = get_field(c, c->parts, rec.data(), rec.size());
+  if (s) return s;
+  for (uint32_t k = 0; k < n; ++k) {
+    const double* r = rec.data() + size_t(k) * kPartFields;
+    std::memset(&p[k], 0, sizeof(lbm_particle));
+    for (int d = 0; d < 3; ++d) {
+      p[k].position[d] = r[PF_POS + d];
+      p[k].position_old[d] = r[PF_POS_OLD + d];
+      p[k].rotation[d] = r[PF_ROT + d];
+      p[k].rotation_old[d] = r[PF_ROT_OLD + d];
+      p[k].acceleration[d] = r[PF_ACC + d];
+      p[k].angular_acceleration[d] = r[PF_ANG_ACC + d];
+      p[k].force[d] = r[PF_FORCE + d];
+      p[k].torque[d] = r[PF_TORQUE + d];
+    }
+    p[k].mass = r[PF_MASS];
+    p[k].inertia = r[PF_INERTIA];
+    p[k].density = r[PF_DENSITY];
+    p[k].radius = r[PF_RADIUS];
+  }
+  return LBM_OK;
+}
+
+lbm_status lbm_psm_couple(lbm_ctx* c, uint64_t t) {
+  (void)t;
+  if (!c) return fail(LBM_ERR_INVALID_ARG, "null ctx");
+  if (c->np == 0) return fail(LBM_ERR_INVALID_STATE, "no particles");
+  HIP_TRY(hipSetDevice(c->device));
+  ParticleArgs a = particle_args(c);
+  HIP_TRY(launch_psm_couple(c->lattice, c->storage, a, c->stream));
+  return LBM_OK;
+}
+
+lbm_status lbm_particle_reduce(lbm_ctx* c, double* force, double* torque) {
+  if (!c) return fail(LBM_ERR_INVALID_ARG, "null ctx");
+  if (c->np == 0) return fail(LBM_ERR_INVALID_STATE, "no particles");
+  HIP_TRY(hipSetDevice(c->device));
+  if (force) HIP_TRY(hipMemcpyAsync(force, c->f64, size_t(c->np) * 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (torque) HIP_TRY(hipMemcpyAsync(torque, c->t64, size_t(c->np) * 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return LBM_OK;
+}
+
+lbm_status lbm_dem_contact(lbm_ctx* c) {
+  if (!c) return fail(LBM_ERR_INVALID_ARG, "null ctx");
+  if (c->np == 0) return fail(LBM_ERR_INVALID_STATE, "no particles");
+  HIP_TRY(hipSetDevice(c->device));
+  ParticleArgs a = particle_args(c);
+  HIP_TRY(launch_dem_contact(c->D, c->storage, a, c->stream));
+  return LBM_OK;
+}
+
+lbm_status lbm_particle_integrate(lbm_ctx* c) {
+  if (!c) return fail(LBM_ERR_INVALID_ARG, "null ctx");
+  if (c->np == 0) return fail(LBM_ERR_INVALID_STATE, "no particles");
+  HIP_TRY(hipSetDevice(c->device));
+  ParticleArgs a = particle_args(c);
+  HIP_TRY(launch_particle_integrate(c->D, c->storage, a, c->stream));
+  return LBM_OK;
+}
+
+lbm_status lbm_cell_list_overflow(lbm_ctx* c, uint64_t* overflow) {
+  if (!c || !overflow) return fail(LBM_ERR_INVALID_ARG, "null argument");
+  *overflow = 0;
+  if (!c->cl_overflow) return LBM_OK;
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipMemcpyAsync(overflow, c->cl_overflow, sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return LBM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,586 @@
+// Particle coupling kernels.
+//
+// One PSM coupling pass (lbm_psm_couple) is, in launch order:
+//   k_clear      porosity := 1 (PsmReset, psm.hpp:20-32), owners := 0
+//   k_owner      per particle and AABB cell: which particle a serial loop over
+//                particles would have written last (highest index), for the
+//                porosity and for the force field
+//   k_gather     per particle and AABB cell: porosity, momentum exchange
+//                (psm.hpp:304-424), owner writes, and the particle's force and
+//                torque summed in FP64 (wavefront shuffles, then LDS, in a
+//                fixed order per block)
+//   k_finalize   per particle: fixed-order sum of the block partials; force
+//                reset + accumulate (integrate.hpp:195-212, psm.hpp:442-443)
+//   [product mode] k_build_cells + k_product: cell list and the per-cell
+//                product porosity (cell_list.hpp:81-150, couple.hpp:26-60)
+// Then k_contact (contact.hpp:466-567) and k_integrate (integrate.hpp:45-83).
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace lbmx {
+
+template <class S, class T>
+__device__ inline T ldp(const S* p, T w);
+template <>
+__device__ inline double ldp<double, double>(const double* p, double) { return *p; }
+template <>
+__device__ inline float ldp<float, float>(const float* p, float) { return *p; }
+template <>
+__device__ inline float ldp<__half, float>(const __half* p, float w) { return __half2float(*p) + w; }
+template <>
+__device__ inline float ldp<__hip_bfloat16, float>(const __hip_bfloat16* p, float w) {
+  return __bfloat162float(*p) + w;
+}
+
+// wrap a coordinate along a periodic axis; false when it leaves the lattice
+__device__ inline bool wrap(int& v, int n, bool periodic) {
+  if (v >= 0 && v < n) return true;
+  if (!periodic) return false;
+  v = (v + n) % n;
+  return true;
+}
+
+// Pre-collision population f_i at (x,y,z) in the current AA phase.
+template <class L, class S, class T>
+__device__ inline T pop_at(const ParticleArgs& a, int i, int x, int y, int z) {
+  const S* f = static_cast<const S*>(a.f);
+  if (a.phase == 0) {
+    const uint32_t c = uint32_t(x) + uint32_t(a.nx) * (uint32_t(y) + uint32_t(a.ny) * uint32_t(z));
+    return ldp<S, T>(f + size_t(i) * a.Np + c, T(L::w[i]));
+  }
+  int xs = x - L::cx[i], ys = y - L::cy[i], zs = z - L::cz[i];
+  if (!wrap(xs, a.nx, a.periodic & 1u) || !wrap(ys, a.ny, a.periodic & 2u) ||
+      !wrap(zs, a.nz, a.periodic & 4u))
+    return T(0);
+  const uint32_t c = uint32_t(xs) + uint32_t(a.nx) * (uint32_t(ys) + uint32_t(a.ny) * uint32_t(zs));
+  return ldp<S, T>(f + size_t(L::opp[i]) * a.Np + c, T(L::w[i]));
+}
+
+template <class T, int D>
+struct Box {
+  int lo[3], hi[3];
+  uint32_t count;
+};
+
+// particle_aabb::try_calculate (aabb.hpp:64-81)
+template <class T, int D>
+__device__ inline Box<T, D> aabb_of(const T* P, const ParticleArgs& a) {
+  Box<T, D> b;
+  const int ext[3] = {a.nx, a.ny, a.nz};
+  const T r = P[PF_RADIUS];
+  b.count = 1;
+  for (int d = 0; d < 3; ++d) {
+    if (d < D) {
+      const T pos = P[PF_POS + d];
+      const T lower = (pos >= r) ? T(pos - r) : T(0);
+      const T upper = pos + r;
+      const T up1 = upper + T(1);
+      // the reference casts to uint64; positions are inside the lattice here
+      uint64_t lo = lower > T(0) ? uint64_t(lower) : 0;
+      uint64_t hi = up1 > T(0) ? uint64_t(up1) : 0;
+      if (hi > uint64_t(ext[d])) hi = uint64_t(ext[d]);
+      if (lo > hi) lo = hi;
+      b.lo[d] = int(lo);
+      b.hi[d] = int(hi);
+    } else {
+      b.lo[d] = 0;
+      b.hi[d] = 1;
+    }
+    b.count *= uint32_t(b.hi[d] - b.lo[d]);
+  }
+  return b;
+}
+
+template <class T, int D>
+__device__ inline void box_cell(const Box<T, D>& b, uint32_t k, int* x) {
+  const uint32_t sz = uint32_t(b.hi[2] - b.lo[2]);
+  const uint32_t sy = uint32_t(b.hi[1] - b.lo[1]);
+  x[2] = b.lo[2] + int(k % sz);
+  k /= sz;
+  x[1] = b.lo[1] + int(k % sy);
+  x[0] = b.lo[0] + int(k / sy);
+}
+
+template <class T, int D>
+__device__ inline T porosity_at(const T* P, const int* x, T eps, T* rel) {
+  for (int d = 0; d < D; ++d) rel[d] = T(x[d]) - P[PF_POS + d];
+  return disc_porosity<T, D>(rel, P[PF_RADIUS], eps);
+}
+
+__global__ void k_clear(float* por_f, double* por_d, uint32_t* owner, uint32_t N, int reset_por) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  if (reset_por) {
+    if (por_d) por_d[c] = 1.0;
+    else por_f[c] = 1.0f;
+  }
+  owner[2 * c] = 0;
+  owner[2 * c + 1] = 0;
+}
+
+template <class L, class T>
+__global__ __launch_bounds__(256) void k_owner(ParticleArgs a) {
+  constexpr int D = L::D;
+  const uint32_t p = blockIdx.x / a.blocks_per_particle;
+  const uint32_t chunk = blockIdx.x % a.blocks_per_particle;
+  if (p >= a.n) return;
+  const T* P = static_cast<const T*>(a.parts) + size_t(p) * kPartFields;
+  const Box<T, D> b = aabb_of<T, D>(P, a);
+  const T eps = T(a.eps);
+  for (uint32_t k = chunk * blockDim.x + threadIdx.x; k < b.count;
+       k += a.blocks_per_particle * blockDim.x) {
+    int x[3];
+    box_cell<T, D>(b, k, x);
+    T rel[3];
+    const T B = porosity_at<T, D>(P, x, eps, rel);
+    const uint32_t c = uint32_t(x[0]) + uint32_t(a.nx) * (uint32_t(x[1]) + uint32_t(a.ny) * uint32_t(x[2]));
+    atomicMax(a.owner + 2 * c, p + 1);
+    if (B < T(1.0)) atomicMax(a.owner + 2 * c + 1, p + 1);
+  }
+}
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+template <class L, class S, class T>
+__global__ __launch_bounds__(256) void k_gather(ParticleArgs a) {
+  constexpr int D = L::D, Q = L::Q;
+  const uint32_t p = blockIdx.x / a.blocks_per_particle;
+  const uint32_t chunk = blockIdx.x % a.blocks_per_particle;
+  if (p >= a.n) return;  // uniform per block
+  const T* P = static_cast<const T*>(a.parts) + size_t(p) * kPartFields;
+  const Box<T, D> b = aabb_of<T, D>(P, a);
+  const T eps = T(a.eps);
+  const T one = T(1.0);
+  // ts = 1/sub_steps, u_p = (X − X_old)/ts (psm.hpp:160-172)
+  const T ts = one / T(a.sub_steps);
+  T vp[3] = {T(0), T(0), T(0)};
+  for (int d = 0; d < D; ++d) vp[d] = (P[PF_POS + d] - P[PF_POS_OLD + d]) / ts;
+  T* por = static_cast<T*>(a.por);
+  T* fout = static_cast<T*>(a.force);
+  const bool scatter = a.porosity_mode == LBM_POROSITY_SCATTER;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (uint32_t k = chunk * blockDim.x + threadIdx.x; k < b.count;
+       k += a.blocks_per_particle * blockDim.x) {
+    int x[3];
+    box_cell<T, D>(b, k, x);
+    T rel[3] = {T(0), T(0), T(0)};
+    const T B = porosity_at<T, D>(P, x, eps, rel);
+    const uint32_t c = uint32_t(x[0]) + uint32_t(a.nx) * (uint32_t(x[1]) + uint32_t(a.ny) * uint32_t(x[2]));
+    if (scatter && a.owner[2 * c] == p + 1) por[c] = B;
+    if (B >= T(1.0)) continue;
+    T mom[3] = {T(0.0), T(0.0), T(0.0)};
+#pragma unroll
+    for (int i = 1; i < Q; ++i) {
+      T upe = T(0);
+#pragma unroll
+      for (int d = 0; d < D; ++d) upe = upe + T(cdir<L>(i, d)) * vp[d];
+      const T fi = pop_at<L, S, T>(a, i, x[0], x[1], x[2]);
+      int xn = x[0] + L::cx[i], yn = x[1] + L::cy[i], zn = x[2] + L::cz[i];
+      T fo = T(0);
+      if (wrap(xn, a.nx, a.periodic & 1u) && wrap(yn, a.ny, a.periodic & 2u) &&
+          wrap(zn, a.nz, a.periodic & 4u))
+        fo = pop_at<L, S, T>(a, L::opp[i], xn, yn, zn);
+      const T added = fi * (T(1.0) - upe) + fo * (T(1.0) + upe);
+#pragma unroll
+      for (int d = 0; d < D; ++d) mom[d] = mom[d] + T(cdir<L>(i, d)) * added;
+    }
+    const T flip = one - B;
+    T fc[3] = {T(0), T(0), T(0)};
+    for (int d = 0; d < D; ++d) fc[d] = mom[d] * flip;
+    if (a.owner[2 * c + 1] == p + 1)
+      for (int d = 0; d < D; ++d) fout[size_t(d) * a.N + c] = fc[d];
+    for (int d = 0; d < D; ++d) acc[d] -= double(fc[d]);
+    if (D == 2) {
+      acc[3] -= double(rel[0] * fc[1] - rel[1] * fc[0]);
+    } else {
+      acc[3] -= double(rel[1] * fc[2] - rel[2] * fc[1]);
+      acc[4] -= double(rel[2] * fc[0] - rel[0] * fc[2]);
+      acc[5] -= double(rel[0] * fc[1] - rel[1] * fc[0]);
+    }
+  }
+  // FP64 reduction: shuffles within the 64-wide wavefront, then the waves
+  // of the block in order through LDS
+  __shared__ double red[4][6];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const double s = wave_sum(acc[k]);
+    if (lane == 0) red[wv][k] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    double s = 0.0;
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int w2 = 0; w2 < nw; ++w2) s += red[w2][threadIdx.x];
+    a.partial[(size_t(p) * a.blocks_per_particle + chunk) * 6 + threadIdx.x] = s;
+  }
+}
+
+template <class T, int D>
+__global__ void k_finalize(ParticleArgs a) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= a.n) return;
+  double s[6] = {0, 0, 0, 0, 0, 0};
+  for (uint32_t k = 0; k < a.blocks_per_particle; ++k)
+    for (int j = 0; j < 6; ++j) s[j] += a.partial[(size_t(p) * a.blocks_per_particle + k) * 6 + j];
+  T* P = static_cast<T*>(a.parts) + size_t(p) * kPartFields;
+  for (int d = 0; d < 3; ++d) {
+    a.f64[p * 3 + d] = s[d];
+    a.t64[p * 3 + d] = s[3 + d];
+    P[PF_FORCE + d] = d < D ? T(s[d]) : T(0);
+    P[PF_TORQUE + d] = T(0);
+  }
+  if (a.psm_torque) {
+    if (D == 2) P[PF_TORQUE] = T(s[3]);
+    else for (int d = 0; d < 3; ++d) P[PF_TORQUE + d] = T(s[3 + d]);
+  }
+}
+
+// build_cell_list (cell_list.hpp:81-108): bins filled in particle-index order
+template <class T, int D>
+__global__ __launch_bounds__(1024) void k_build_cells(ParticleArgs a, uint64_t bins) {
+  __shared__ uint32_t bin_of[4096];
+  for (uint64_t k = threadIdx.x; k < bins; k += blockDim.x) a.cl_count[k] = 0;
+  if (threadIdx.x == 0) *a.cl_overflow = 0;
+  const T bin = T(a.bin);
+  for (uint32_t p = threadIdx.x; p < a.n; p += blockDim.x) {
+    const T* P = static_cast<const T*>(a.parts) + size_t(p) * kPartFields;
+    uint64_t flat = 0, stride = 1;
+    for (int d = 0; d < D; ++d) {
+      const T xx = P[PF_POS + d] / bin;
+      int64_t bb = int64_t(xx);
+      if (bb < 0) bb = 0;
+      if (uint64_t(bb) >= a.grid[d]) bb = int64_t(a.grid[d]) - 1;
+      flat += uint64_t(bb) * stride;
+      stride *= a.grid[d];
+    }
+    bin_of[p] = uint32_t(flat);
+  }
+  __syncthreads();
+  for (uint32_t p = threadIdx.x; p < a.n; p += blockDim.x) {
+    const uint32_t bp = bin_of[p];
+    uint32_t rank = 0;
+    for (uint32_t q = 0; q < p; ++q) rank += (bin_of[q] == bp);
+    if (rank < a.cap) {
+      a.cl_slot[uint64_t(bp) * a.cap + rank] = p;
+      atomicMax((unsigned long long*)(a.cl_count + bp), (unsigned long long)(rank + 1));
+    } else {
+      atomicAdd((unsigned long long*)a.cl_overflow, 1ull);
+    }
+  }
+}
+
+// cell_porosity (couple.hpp:26-60) with for_each_bin_candidate's order
+template <class L, class T>
+__global__ __launch_bounds__(256) void k_product(ParticleArgs a) {
+  constexpr int D = L::D;
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.N) return;
+  const int x[3] = {int(c % uint32_t(a.nx)), int((c / uint32_t(a.nx)) % uint32_t(a.ny)),
+                    int(c / (uint32_t(a.nx) * uint32_t(a.ny)))};
+  const T bin = T(a.bin);
+  const T eps = T(a.eps);
+  int64_t base[3];
+  for (int d = 0; d < D; ++d) base[d] = int64_t(T(x[d]) / bin);
+  T B = T(1);
+  uint32_t total = 1;
+  for (int d = 0; d < D; ++d) total *= 3u;
+  for (uint32_t off = 0; off < total; ++off) {
+    uint32_t rem = off;
+    uint64_t flat = 0, stride = 1;
+    bool inside = true;
+    for (int d = 0; d < D; ++d) {
+      const int64_t bb = base[d] + int64_t(rem % 3u) - 1;
+      rem /= 3u;
+      if (bb < 0 || bb >= int64_t(a.grid[d])) {
+        inside = false;
+        break;
+      }
+      flat += uint64_t(bb) * stride;
+      stride *= a.grid[d];
+    }
+    if (!inside) continue;
+    const uint64_t cnt = a.cl_count[flat];
+    for (uint64_t s = 0; s < cnt; ++s) {
+      const uint64_t q = a.cl_slot[flat * a.cap + s];
+      const T* P = static_cast<const T*>(a.parts) + size_t(q) * kPartFields;
+      T rel[3];
+      for (int d = 0; d < D; ++d) rel[d] = T(x[d]) - P[PF_POS + d];
+      B = B * disc_porosity<T, D>(rel, P[PF_RADIUS], eps);
+    }
+  }
+  static_cast<T*>(a.por)[c] = B;
+}
+
+// ---------------------------------------------------------------- DEM
+template <class T, int D>
+struct Man {
+  T depth, n[3], pt[3];
+};
+
+template <class T, int D>
+__device__ inline bool safe_norm(const T* in, T* out) {
+  T l2 = T(0);
+  for (int d = 0; d < D; ++d) l2 = l2 + in[d] * in[d];
+  if (!(l2 > T(0))) return false;
+  const T inv = T(1) / sqrt(l2);
+  for (int d = 0; d < D; ++d) out[d] = in[d] * inv;
+  return true;
+}
+
+template <class T, int D>
+__device__ inline void cross_rf(const T* r, const T* f, T* out) {
+  if (D == 2) {
+    out[0] = r[0] * f[1] - r[1] * f[0];
+  } else {
+    out[0] = r[1] * f[2] - r[2] * f[1];
+    out[1] = r[2] * f[0] - r[0] * f[2];
+    out[2] = r[0] * f[1] - r[1] * f[0];
+  }
+}
+
+// narrow_phase<Spheroid>::contact (contact.hpp:124-184)
+template <class T, int D>
+__device__ inline bool narrow(const T* lo, const T* hi, Man<T, D>& m) {
+  T d[3];
+  for (int k = 0; k < D; ++k) d[k] = lo[PF_POS + k] - hi[PF_POS + k];
+  T len2 = T(0);
+  for (int k = 0; k < D; ++k) len2 = len2 + d[k] * d[k];
+  const T r_lo = lo[PF_RADIUS], r_hi = hi[PF_RADIUS];
+  const T sum_r = r_lo + r_hi;
+  if (!(len2 < sum_r * sum_r)) return false;
+  const T len = sqrt(len2);
+  T depth = sum_r - len;
+  const T cap = (r_lo < r_hi) ? r_lo : r_hi;
+  if (depth > cap) depth = cap;
+  if (!safe_norm<T, D>(d, m.n)) {
+    for (int k = 0; k < D; ++k) m.n[k] = T(0);
+    m.n[0] = T(1);
+  }
+  m.depth = depth;
+  for (int k = 0; k < D; ++k) m.pt[k] = T(hi[PF_POS + k] + m.n[k] * (r_hi - depth * 0.5));
+  return true;
+}
+
+// impl::surface_velocity (contact.hpp:271-311)
+template <class T, int D>
+__device__ inline void surf_vel(const T* P, const T* r, T steps, T* v) {
+  for (int k = 0; k < D; ++k) v[k] = (P[PF_POS + k] - P[PF_POS_OLD + k]) * steps;
+  if (D == 2) {
+    const T om = (P[PF_ROT] - P[PF_ROT_OLD]) * steps;
+    const T t0 = -(om * r[1]), t1 = om * r[0];
+    v[0] = v[0] + t0;
+    v[1] = v[1] + t1;
+  } else {
+    T om[3], tg[3];
+    for (int k = 0; k < 3; ++k) om[k] = (P[PF_ROT + k] - P[PF_ROT_OLD + k]) * steps;
+    cross_rf<T, 3>(om, r, tg);
+    for (int k = 0; k < 3; ++k) v[k] = v[k] + tg[k];
+  }
+}
+
+// impl::soft_shell_response (contact.hpp:204-264); the damping and normal
+// force carry double literals in the reference and are double here too
+template <class T, int D>
+__device__ inline void soft_shell(const Man<T, D>& m, const T* r_self, const T* v_rel, T m_eff,
+                                  const ParticleArgs& a, T* F, T* Tq) {
+  const T k_n = T(a.k_n), zeta = T(a.zeta_n), gamma_t = T(a.gamma_t), mu = T(a.mu);
+  const double g_n = 2.0 * double(zeta) * double(T(sqrt(T(m_eff * k_n))));
+  T v_n = T(0);
+  for (int k = 0; k < D; ++k) v_n = v_n + v_rel[k] * m.n[k];
+  double f_n = double(T(k_n * m.depth)) - g_n * double(v_n);
+  if (f_n < 0) f_n = 0;
+  T v_t[3], t_hat[3];
+  for (int k = 0; k < D; ++k) v_t[k] = v_rel[k] - v_n * m.n[k];
+  const bool have_t = safe_norm<T, D>(v_t, t_hat);
+  T vt2 = T(0);
+  for (int k = 0; k < D; ++k) vt2 = vt2 + v_t[k] * v_t[k];
+  const T v_t_len = sqrt(vt2);
+  T f_t = gamma_t * v_t_len;
+  const double f_t_cap = double(mu) * f_n;
+  if (double(f_t) > f_t_cap) f_t = T(f_t_cap);
+  for (int k = 0; k < D; ++k) {
+    double v = f_n * double(m.n[k]);
+    if (have_t) v -= double(T(f_t * t_hat[k]));
+    F[k] = T(v);
+  }
+  cross_rf<T, D>(r_self, F, Tq);
+}
+
+template <class T, int D>
+__global__ void k_contact(ParticleArgs a) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.n) return;
+  T* parts = static_cast<T*>(a.parts);
+  T* me = parts + size_t(s) * kPartFields;
+  const T steps = T(a.sub_steps == 0 ? 1 : a.sub_steps);
+  T Fs[3] = {me[PF_FORCE], me[PF_FORCE + 1], me[PF_FORCE + 2]};
+  T Ts = me[PF_TORQUE];
+  for (uint32_t j = 0; j < a.n; ++j) {
+    if (j == s) continue;
+    const T* ot = parts + size_t(j) * kPartFields;
+    // broadphase_collision_check (particle.hpp:247-279)
+    T n2 = T(0);
+    for (int k = 0; k < D; ++k) {
+      const T dd = me[PF_POS + k] - ot[PF_POS + k];
+      n2 = n2 + dd * dd;
+    }
+    const T rl = me[PF_RADIUS], rr = ot[PF_RADIUS];
+    const T rab2 = (rl * rl + rr * rr) + (rr * rl) * T(2.0);
+    if (!(n2 < rab2)) continue;
+    const bool self_lo = s < j;
+    Man<T, D> m;
+    if (!narrow<T, D>(self_lo ? me : ot, self_lo ? ot : me, m)) continue;
+    // particle_contact_response (contact.hpp:321-364)
+    if (!self_lo)
+      for (int k = 0; k < D; ++k) m.n[k] = -m.n[k];
+    T r_s[3], r_o[3], v_s[3], v_o[3], v_rel[3];
+    for (int k = 0; k < D; ++k) {
+      r_s[k] = m.pt[k] - me[PF_POS + k];
+      r_o[k] = m.pt[k] - ot[PF_POS + k];
+    }
+    surf_vel<T, D>(me, r_s, steps, v_s);
+    surf_vel<T, D>(ot, r_o, steps, v_o);
+    for (int k = 0; k < D; ++k) v_rel[k] = v_s[k] - v_o[k];
+    const T m_eff = (me[PF_MASS] * ot[PF_MASS]) / (me[PF_MASS] + ot[PF_MASS]);
+    T F[3] = {T(0), T(0), T(0)}, Tq[3] = {T(0), T(0), T(0)};
+    soft_shell<T, D>(m, r_s, v_rel, m_eff, a, F, Tq);
+    for (int k = 0; k < D; ++k) Fs[k] = Fs[k] + F[k];
+    if (D == 2) Ts = Ts + Tq[0];
+  }
+  if (a.walls) {  // accumulate_box_wall_contacts (contact.hpp:407-456)
+    const int ext[3] = {a.nx, a.ny, a.nz};
+    const T r = me[PF_RADIUS];
+    for (int ax = 0; ax < D; ++ax) {
+      const T x = me[PF_POS + ax];
+      const T hb = T(ext[ax]);
+      for (int side = 0; side < 2; ++side) {
+        T depth = (side == 0) ? T(r - x) : T(x - (hb - r));
+        if (!(depth > T(0))) continue;
+        if (depth > r) depth = r;
+        Man<T, D> m;
+        for (int k = 0; k < D; ++k) {
+          m.n[k] = T(0);
+          m.pt[k] = me[PF_POS + k];
+        }
+        m.n[ax] = (side == 0) ? T(1.0) : T(-1.0);
+        m.pt[ax] = (side == 0) ? T(0.0) : hb;
+        m.depth = depth;
+        T r_s[3], v_rel[3];
+        for (int k = 0; k < D; ++k) r_s[k] = m.pt[k] - me[PF_POS + k];
+        surf_vel<T, D>(me, r_s, steps, v_rel);
+        T F[3] = {T(0), T(0), T(0)}, Tq[3] = {T(0), T(0), T(0)};
+        soft_shell<T, D>(m, r_s, v_rel, me[PF_MASS], a, F, Tq);
+        for (int k = 0; k < D; ++k) Fs[k] = Fs[k] + F[k];
+        if (D == 2) Ts = Ts + Tq[0];
+      }
+    }
+  }
+  // only force/torque of this particle are written; no thread reads them
+  for (int k = 0; k < D; ++k) me[PF_FORCE + k] = Fs[k];
+  if (D == 2) me[PF_TORQUE] = Ts;
+}
+
+// integrate_particle + verlet_step_lambda (integrate.hpp:45-83, particle.hpp:171-211)
+template <class T, int D>
+__global__ void k_integrate(ParticleArgs a) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.n) return;
+  T* P = static_cast<T*>(a.parts) + size_t(s) * kPartFields;
+  const uint64_t steps = a.sub_steps == 0 ? 1 : a.sub_steps;
+  const T ts = T(1) / T(steps);
+  for (int k = 0; k < D; ++k) P[PF_ACC + k] = P[PF_FORCE + k] / P[PF_MASS] + T(a.g[k]);
+  if (D == 2) P[PF_ANG_ACC] = P[PF_TORQUE] / P[PF_INERTIA];
+  constexpr int R = (D == 2) ? 1 : 3;
+  for (uint64_t st = 0; st < steps; ++st) {
+    const T ts2 = ts * ts;
+    for (int k = 0; k < D; ++k) {
+      const T pn = ((P[PF_POS + k] * T(2.0)) - P[PF_POS_OLD + k]) + P[PF_ACC + k] * ts2;
+      P[PF_POS_OLD + k] = P[PF_POS + k];
+      P[PF_POS + k] = pn;
+    }
+    for (int k = 0; k < R; ++k) {
+      const T rn = ((P[PF_ROT + k] * T(2.0)) - P[PF_ROT_OLD + k]) + P[PF_ANG_ACC + k] * ts2;
+      P[PF_ROT_OLD + k] = P[PF_ROT + k];
+      P[PF_ROT + k] = rn;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- launchers
+template <class L, class S, class T>
+static hipError_t couple_t(const ParticleArgs& a, hipStream_t s) {
+  const bool dbl = sizeof(T) == 8;
+  const uint32_t N = a.N;
+  const bool scatter = a.porosity_mode == LBM_POROSITY_SCATTER;
+  hipLaunchKernelGGL(k_clear, dim3((N + 255) / 256), dim3(256), 0, s, dbl ? nullptr : (float*)a.por,
+                     dbl ? (double*)a.por : nullptr, a.owner, N, scatter ? 1 : 0);
+  const dim3 pg(a.n * a.blocks_per_particle);
+  hipLaunchKernelGGL((k_owner<L, T>), pg, dim3(256), 0, s, a);
+  hipLaunchKernelGGL((k_gather<L, S, T>), pg, dim3(256), 0, s, a);
+  hipLaunchKernelGGL((k_finalize<T, L::D>), dim3((a.n + 255) / 256), dim3(256), 0, s, a);
+  if (!scatter) {
+    uint64_t bins = a.grid[0] * a.grid[1] * a.grid[2];
+    hipLaunchKernelGGL((k_build_cells<T, L::D>), dim3(1), dim3(1024), 0, s, a, bins);
+    hipLaunchKernelGGL((k_product<L, T>), dim3((N + 255) / 256), dim3(256), 0, s, a);
+  }
+  return hipGetLastError();
+}
+
+template <class L>
+static hipError_t couple_l(int storage, const ParticleArgs& a, hipStream_t s) {
+  switch (storage) {
+    case LBM_F64: return couple_t<L, double, double>(a, s);
+    case LBM_F32: return couple_t<L, float, float>(a, s);
+    case LBM_F16S: return couple_t<L, __half, float>(a, s);
+    case LBM_BF16S: return couple_t<L, __hip_bfloat16, float>(a, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_psm_couple(int lattice, int storage, const ParticleArgs& a, hipStream_t s) {
+  if (a.n == 0) return hipSuccess;
+  switch (lattice) {
+    case LBM_D2Q9: return couple_l<D2Q9>(storage, a, s);
+    case LBM_D3Q19: return couple_l<D3Q19>(storage, a, s);
+    case LBM_D3Q27: return couple_l<D3Q27>(storage, a, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_dem_contact(int dims, int storage, const ParticleArgs& a, hipStream_t s) {
+  if (a.n == 0) return hipSuccess;
+  const dim3 g((a.n + 255) / 256), b(256);
+  const bool dbl = storage == LBM_F64;
+  if (dims == 2) {
+    if (dbl) hipLaunchKernelGGL((k_contact<double, 2>), g, b, 0, s, a);
+    else hipLaunchKernelGGL((k_contact<float, 2>), g, b, 0, s, a);
+  } else {
+    if (dbl) hipLaunchKernelGGL((k_contact<double, 3>), g, b, 0, s, a);
+    else hipLaunchKernelGGL((k_contact<float, 3>), g, b, 0, s, a);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_particle_integrate(int dims, int storage, const ParticleArgs& a, hipStream_t s) {
+  if (a.n == 0) return hipSuccess;
+  const dim3 g((a.n + 255) / 256), b(256);
+  const bool dbl = storage == LBM_F64;
+  if (dims == 2) {
+    if (dbl) hipLaunchKernelGGL((k_integrate<double, 2>), g, b, 0, s, a);
+    else hipLaunchKernelGGL((k_integrate<float, 2>), g, b, 0, s, a);
+  } else {
+    if (dbl) hipLaunchKernelGGL((k_integrate<double, 3>), g, b, 0, s, a);
+    else hipLaunchKernelGGL((k_integrate<float, 3>), g, b, 0, s, a);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace lbmx

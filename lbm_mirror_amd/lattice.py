@@ -1,0 +1,163 @@
+"""Lattice: one GPU lattice behind the C ABI (include/lbm_mirror_amd/lbm.h).
+
+The reference drives its components from a per-cell `switch(info)` inside a
+SYCL kernel (examples/poiseulle_2d_gpu/poiseulle_2d_gpu.cpp:193-222).  Here
+the same mapping flag -> (boundary component, collision component) is
+registered once with ``set_flag`` and ``step`` advances whole steps on the GPU.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import abi, check, load_library
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Lattice:
+    def __init__(self, lattice, storage, extent, device=0):
+        self.L = load_library()
+        self.lattice, self.storage = lattice, storage
+        self.D, self.Q = abi.LATTICE_DQ[lattice]
+        ext = list(extent) + [1] * (3 - len(extent))
+        self.extent = tuple(ext[: self.D])
+        self.N = int(np.prod(self.extent))
+        h = C.c_void_p()
+        check(self.L.lbm_ctx_create(C.byref(h), lattice, storage, (C.c_uint64 * 3)(*ext), device))
+        self._h = h
+        self.t = 0
+        self.n_particles = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.L.lbm_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- set-up -----------------------------------------------------------
+    def set_flags(self, flags):
+        f = np.ascontiguousarray(flags, dtype=np.uint8).reshape(-1)
+        if f.size != self.N:
+            raise ValueError(f"flags: expected {self.N} cells, got {f.size}")
+        check(self.L.lbm_set_flags(self._h, f.ctypes.data_as(C.POINTER(C.c_uint8))))
+
+    def set_periodic(self, mask):
+        check(self.L.lbm_set_periodic(self._h, mask))
+
+    def set_operator(self, flag, op):
+        check(self.L.lbm_set_operator(self._h, flag, C.byref(op)))
+
+    def set_flag(self, flag, boundary=None, collision=None):
+        """Register what the drivers' switch does for ``flag``: an optional
+        boundary component, then an optional collision component."""
+        from .components import combine
+        self.set_operator(flag, combine(boundary, collision))
+
+    def set_boundary_velocity(self, u):
+        a = np.ascontiguousarray(u, dtype=np.float64).reshape(-1)
+        if a.size != self.D * self.N:
+            raise ValueError("boundary velocity: expected D*N values")
+        check(self.L.lbm_set_boundary_velocity(self._h, _dp(a)))
+
+    def set_force_field(self, f):
+        a = np.ascontiguousarray(f, dtype=np.float64).reshape(-1)
+        check(self.L.lbm_set_force_field(self._h, _dp(a)))
+
+    def get_force_field(self):
+        a = np.zeros(self.D * self.N)
+        check(self.L.lbm_get_force_field(self._h, _dp(a)))
+        return a.reshape(self.D, self.N)
+
+    def set_porosity(self, p):
+        a = np.ascontiguousarray(p, dtype=np.float64).reshape(-1)
+        check(self.L.lbm_set_porosity(self._h, _dp(a)))
+
+    def get_porosity(self):
+        a = np.zeros(self.N)
+        check(self.L.lbm_get_porosity(self._h, _dp(a)))
+        return a
+
+    def set_populations(self, f):
+        a = np.ascontiguousarray(f, dtype=np.float64).reshape(-1)
+        if a.size != self.Q * self.N:
+            raise ValueError("populations: expected Q*N values")
+        check(self.L.lbm_set_populations(self._h, _dp(a)))
+
+    def get_populations(self):
+        a = np.zeros(self.Q * self.N)
+        check(self.L.lbm_get_populations(self._h, _dp(a)))
+        return a.reshape(self.Q, self.N)
+
+    def init_equilibrium(self, rho=1.0, u=(0.0, 0.0, 0.0)):
+        uu = np.zeros(3)
+        uu[: len(u)] = u
+        check(self.L.lbm_init_equilibrium(self._h, rho, _dp(uu)))
+
+    def get_macros(self):
+        r = np.zeros(self.N)
+        u = np.zeros(self.D * self.N)
+        check(self.L.lbm_get_macros(self._h, _dp(r), _dp(u)))
+        return r, u.reshape(self.D, self.N)
+
+    def set_macros(self, rho, u):
+        r = np.ascontiguousarray(rho, dtype=np.float64).reshape(-1)
+        uu = np.ascontiguousarray(u, dtype=np.float64).reshape(-1)
+        check(self.L.lbm_set_macros(self._h, _dp(r), _dp(uu)))
+
+    # ---- stepping ---------------------------------------------------------
+    def step(self, n=1, t0=None):
+        t = self.t if t0 is None else t0
+        check(self.L.lbm_step(self._h, t, n))
+        self.t = t + n
+
+    def sync(self):
+        check(self.L.lbm_sync(self._h))
+
+    def last_step_ms(self):
+        m = C.c_double()
+        check(self.L.lbm_last_step_ms(self._h, C.byref(m)))
+        return m.value
+
+    # ---- particles --------------------------------------------------------
+    def particles_set(self, parts, params):
+        arr = (abi.Particle * max(1, len(parts)))(*parts)
+        check(self.L.lbm_particles_set(self._h, arr, len(parts), C.byref(params)))
+        self.n_particles = len(parts)
+
+    def particles_get(self):
+        arr = (abi.Particle * max(1, self.n_particles))()
+        check(self.L.lbm_particles_get(self._h, arr, self.n_particles))
+        return list(arr)[: self.n_particles]
+
+    def psm_couple(self):
+        check(self.L.lbm_psm_couple(self._h, self.t))
+        n = self.n_particles
+        f = np.zeros(3 * n)
+        t = np.zeros(3 * n)
+        check(self.L.lbm_particle_reduce(self._h, _dp(f), _dp(t)))
+        return f.reshape(n, 3), t.reshape(n, 3)
+
+    def dem_contact(self):
+        check(self.L.lbm_dem_contact(self._h))
+
+    def integrate(self):
+        check(self.L.lbm_particle_integrate(self._h))
+
+    def overflow(self):
+        o = C.c_uint64()
+        check(self.L.lbm_cell_list_overflow(self._h, C.byref(o)))
+        return o.value
+
+
+def spheroid_mass(dims, storage, radius, density):
+    L = load_library()
+    m, i = C.c_double(), C.c_double()
+    check(L.lbm_spheroid_mass(dims, storage, radius, density, C.byref(m), C.byref(i)))
+    return m.value, i.value
