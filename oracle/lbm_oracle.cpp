@@ -589,7 +589,8 @@ struct Sim final : SimBase {
     has_op[flag] = true;
   }
   void set_boundary_velocity(const double* u) override {
-    for (uint64_t i = 0; i < N * D; ++i) bc_vel[i] = T(u[i]);
+    for (uint64_t c = 0; c < N; ++c)
+      for (int d = 0; d < D; ++d) bc_vel[c * D + d] = T(u[d * N + c]);
   }
   // fields at the boundary are d*N + cell; stored cell*D + d
   void set_force(const double* f) override {

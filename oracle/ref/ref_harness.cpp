@@ -1,0 +1,291 @@
+// ref_harness.cpp — calls the reference's own host-side functions, compiled
+// in place from /root/reference (see Makefile), so the oracle can be checked
+// against the real arithmetic.  TEST INFRASTRUCTURE ONLY: built into
+// oracle/_ref/, loaded by tests/ only.
+//
+// Covered (all on the reference's host `data<>` types, the way its own tests
+// in modules/core/tests call them):
+//   equilibrium           modules/core/c++/equilibrium.hpp:7-48
+//   compute_rho_u         modules/core/c++/macroscopic.hpp:10-35
+//   particle_porosity     modules/core/c++/particle/porosity.hpp:56-89
+//   particle_aabb         modules/core/c++/particle/aabb.hpp:41-99
+//   create_spheroid_particle_group mass/inertia  particle/particle.hpp:19-116
+//   narrow_phase / particle_contact_response / accumulate_particle_contacts /
+//   accumulate_box_wall_contacts  particle/contact.hpp:124-512
+//   reset_particle_accumulators / integrate_particle  particle/integrate.hpp
+//   make_cell_list / build_cell_list / cell_porosity  particle/cell_list.hpp,
+//                                                       particle/couple.hpp
+// The fluid components (BGK, BGKGuo, Stream, Zou/He, Psm, the PSM gather)
+// take Chunk field views that only the SYCL backend provides
+// (modules/sycl/c++/data.hpp:323-396); AdaptiveCpp is absent, so they are not
+// built here.
+#include <cstdint>
+#include <cstring>
+
+#include "modules/core/c++/equilibrium.hpp"
+#include "modules/core/c++/macroscopic.hpp"
+#include "modules/core/c++/particle/contact.hpp"
+#include "modules/core/c++/particle/couple.hpp"
+#include "modules/core/c++/particle/integrate.hpp"
+#include "modules/core/c++/particle/particle.hpp"
+#include "modules/core/c++/particle/porosity.hpp"
+
+#include "../../include/lbm_mirror_amd/lbm.h"
+
+namespace kl = kel::lbm;
+namespace sch = kel::lbm::sch;
+
+namespace {
+
+template <class T, class Desc>
+void eq_t(double rho_in, const double* u, double* out) {
+  kl::data<sch::Scalar<T>> rho;
+  rho.at({}).set(rho_in);
+  kl::data<sch::Vector<T, Desc::D>> vel;
+  for (uint64_t d = 0; d < Desc::D; ++d) vel.at({{d}}).set(u[d]);
+  auto eq = kl::equilibrium<T, Desc>(rho, vel);
+  for (uint64_t i = 0; i < Desc::Q; ++i) out[i] = eq.at(i).get();
+}
+
+template <class T, class Desc>
+void rhou_t(const double* f, double* rho_out, double* u_out) {
+  kl::data<sch::FixedArray<T, Desc::Q>> dfs;
+  for (uint64_t i = 0; i < Desc::Q; ++i) dfs.at({i}).set(f[i]);
+  kl::data<sch::Scalar<T>> rho;
+  kl::data<sch::Vector<T, Desc::D>> vel;
+  kl::compute_rho_u<T, Desc>(dfs, rho, vel);
+  *rho_out = rho.at({}).get();
+  for (uint64_t d = 0; d < Desc::D; ++d) u_out[d] = vel.at({{d}}).get();
+}
+
+template <class T, uint64_t D>
+double porosity_t(const double* rel, double r, double eps_in) {
+  kl::data<sch::Vector<T, D>> v;
+  for (uint64_t d = 0; d < D; ++d) v.at({{d}}).set(rel[d]);
+  kl::data<sch::Scalar<T>> rad, eps;
+  rad.at({}).set(r);
+  eps.at({}).set(eps_in);
+  auto p = kl::particle_porosity<T, D, 1u, kl::por::ParticleSpheroid<T>>::calculate(v, rad, eps);
+  return p.at({}).get();
+}
+
+template <class T, uint64_t D, uint64_t PC>
+kl::data<sch::ParticleGroup<T, D, PC, kl::coll::Spheroid<T>>> make_group(const lbm_particle* p) {
+  kl::data<sch::LbmMeter<sch::Scalar<T>>> r;
+  kl::data<sch::Scalar<T>> dens;
+  r.at({}).set(p[0].radius);
+  dens.at({}).set(p[0].density);
+  auto g = kl::create_spheroid_particle_group<T, D, PC>(r, dens, {4u});
+  auto& parts = g.template get<"particles">();
+  for (uint64_t k = 0; k < PC; ++k) {
+    auto& q = parts.at({k});
+    auto& rb = q.template get<"rigid_body">();
+    for (uint64_t d = 0; d < D; ++d) {
+      rb.template get<"position">().at({{d}}).set(p[k].position[d]);
+      rb.template get<"position_old">().at({{d}}).set(p[k].position_old[d]);
+      rb.template get<"acceleration">().at({{d}}).set(p[k].acceleration[d]);
+      rb.template get<"force">().at({{d}}).set(p[k].force[d]);
+    }
+    if constexpr (D == 2) {
+      rb.template get<"rotation">().at({}).set(p[k].rotation[0]);
+      rb.template get<"rotation_old">().at({}).set(p[k].rotation_old[0]);
+      rb.template get<"angular_acceleration">().at({}).set(p[k].angular_acceleration[0]);
+      rb.template get<"torque">().at({}).set(p[k].torque[0]);
+    } else {
+      for (uint64_t d = 0; d < 3; ++d) {
+        rb.template get<"rotation">().at({{d}}).set(p[k].rotation[d]);
+        rb.template get<"rotation_old">().at({{d}}).set(p[k].rotation_old[d]);
+        rb.template get<"angular_acceleration">().at({{d}}).set(p[k].angular_acceleration[d]);
+        rb.template get<"torque">().at({{d}}).set(p[k].torque[d]);
+      }
+    }
+    rb.template get<"mass">().at({}).set(p[k].mass);
+    rb.template get<"inertia">().at({}).set(p[k].inertia);
+    rb.template get<"density">().at({}).set(p[k].density);
+    q.template get<"collision">().template get<"radius">().at({}).set(p[k].radius);
+  }
+  return g;
+}
+
+template <class T, uint64_t D, class G>
+void read_group(const G& g, lbm_particle* p, uint64_t PC) {
+  auto& parts = g.template get<"particles">();
+  for (uint64_t k = 0; k < PC; ++k) {
+    auto& q = parts.at({k});
+    auto& rb = q.template get<"rigid_body">();
+    std::memset(&p[k], 0, sizeof(lbm_particle));
+    for (uint64_t d = 0; d < D; ++d) {
+      p[k].position[d] = rb.template get<"position">().at({{d}}).get();
+      p[k].position_old[d] = rb.template get<"position_old">().at({{d}}).get();
+      p[k].acceleration[d] = rb.template get<"acceleration">().at({{d}}).get();
+      p[k].force[d] = rb.template get<"force">().at({{d}}).get();
+    }
+    if constexpr (D == 2) {
+      p[k].rotation[0] = rb.template get<"rotation">().at({}).get();
+      p[k].rotation_old[0] = rb.template get<"rotation_old">().at({}).get();
+      p[k].angular_acceleration[0] = rb.template get<"angular_acceleration">().at({}).get();
+      p[k].torque[0] = rb.template get<"torque">().at({}).get();
+    } else {
+      for (uint64_t d = 0; d < 3; ++d) {
+        p[k].rotation[d] = rb.template get<"rotation">().at({{d}}).get();
+        p[k].rotation_old[d] = rb.template get<"rotation_old">().at({{d}}).get();
+        p[k].angular_acceleration[d] = rb.template get<"angular_acceleration">().at({{d}}).get();
+        p[k].torque[d] = rb.template get<"torque">().at({{d}}).get();
+      }
+    }
+    p[k].mass = rb.template get<"mass">().at({}).get();
+    p[k].inertia = rb.template get<"inertia">().at({}).get();
+    p[k].density = rb.template get<"density">().at({}).get();
+    p[k].radius = q.template get<"collision">().template get<"radius">().at({}).get();
+  }
+}
+
+// One DEM phase sequence on a group: optional reset, contacts for every
+// particle (plus box walls), then optional integration — the order of the
+// reference's own dynamics tests (contact_dynamics.cpp:44-61).
+template <class T, uint64_t D, uint64_t PC>
+void dem_t(lbm_particle* p, const lbm_particle_params* prm, const uint64_t* extent, int reset,
+           int contact, int integrate) {
+  auto g = make_group<T, D, PC>(p);
+  auto& parts = g.template get<"particles">();
+  kl::dem_params<T> dem{prm->k_n, prm->zeta_n, prm->gamma_t, prm->mu};
+  kl::data<sch::FixedArray<sch::UInt64, D>> ext;
+  for (uint64_t d = 0; d < D; ++d) ext.at({d}).set(extent[d]);
+  if (reset)
+    for (uint64_t k = 0; k < PC; ++k) kl::reset_particle_accumulators<T, D>(parts.at({k}));
+  if (contact) {
+    for (uint64_t k = 0; k < PC; ++k) {
+      kl::accumulate_particle_contacts<T, D>(parts, k, dem, {prm->sub_steps});
+      if (prm->walls) kl::accumulate_box_wall_contacts<T, D>(parts.at({k}), ext, dem, {prm->sub_steps});
+    }
+  }
+  if (integrate) {
+    kl::data<sch::LbmAcceleration<sch::Vector<T, D>>> acc;
+    for (uint64_t d = 0; d < D; ++d) acc.at({{d}}).set(prm->body_acceleration[d]);
+    for (uint64_t k = 0; k < PC; ++k) kl::integrate_particle<T, D>(parts.at({k}), acc, {prm->sub_steps});
+  }
+  read_group<T, D>(g, p, PC);
+}
+
+template <class T, uint64_t D>
+int dem_n(uint32_t n, lbm_particle* p, const lbm_particle_params* prm, const uint64_t* ext,
+          int reset, int contact, int integrate) {
+  switch (n) {
+    case 1: dem_t<T, D, 1>(p, prm, ext, reset, contact, integrate); return 0;
+    case 2: dem_t<T, D, 2>(p, prm, ext, reset, contact, integrate); return 0;
+    case 3: dem_t<T, D, 3>(p, prm, ext, reset, contact, integrate); return 0;
+    case 4: dem_t<T, D, 4>(p, prm, ext, reset, contact, integrate); return 0;
+  }
+  return -1;
+}
+
+// cell_porosity over every cell of a 2D lattice, capacity 4
+template <class T, uint64_t PC>
+uint64_t couple_t(const lbm_particle* p, const uint64_t* extent, double max_radius, double eps_in,
+                  double* out) {
+  auto g = make_group<T, 2, PC>(p);
+  auto& parts = g.template get<"particles">();
+  kl::data<sch::FixedArray<sch::UInt64, 2>> ext;
+  for (uint64_t d = 0; d < 2; ++d) ext.at({d}).set(extent[d]);
+  auto cl = kl::make_cell_list<T, 2, 4u>(ext, max_radius);
+  kl::build_cell_list<T, 2, 4u>(parts, cl);
+  kl::data<sch::Scalar<T>> eps;
+  eps.at({}).set(eps_in);
+  for (uint64_t y = 0; y < extent[1]; ++y)
+    for (uint64_t x = 0; x < extent[0]; ++x) {
+      kl::data<sch::FixedArray<sch::UInt64, 2>> cell{{x, y}};
+      out[x + extent[0] * y] = kl::cell_porosity<T, 2, 4u>(parts, cl, cell, eps).at({}).get();
+    }
+  return cl.template get<"overflow">().at({0u}).get();
+}
+
+}  // namespace
+
+extern "C" {
+
+int ref_equilibrium(int lat, int st, double rho, const double* u, double* eq) {
+  if (lat == LBM_D2Q9) st == LBM_F64 ? eq_t<sch::Float64, sch::D2Q9>(rho, u, eq) : eq_t<sch::Float32, sch::D2Q9>(rho, u, eq);
+  else if (lat == LBM_D3Q19) st == LBM_F64 ? eq_t<sch::Float64, sch::D3Q19>(rho, u, eq) : eq_t<sch::Float32, sch::D3Q19>(rho, u, eq);
+  else if (lat == LBM_D3Q27) st == LBM_F64 ? eq_t<sch::Float64, sch::D3Q27>(rho, u, eq) : eq_t<sch::Float32, sch::D3Q27>(rho, u, eq);
+  else return -1;
+  return 0;
+}
+
+int ref_rho_u(int lat, int st, const double* f, double* rho, double* u) {
+  if (lat == LBM_D2Q9) st == LBM_F64 ? rhou_t<sch::Float64, sch::D2Q9>(f, rho, u) : rhou_t<sch::Float32, sch::D2Q9>(f, rho, u);
+  else if (lat == LBM_D3Q19) st == LBM_F64 ? rhou_t<sch::Float64, sch::D3Q19>(f, rho, u) : rhou_t<sch::Float32, sch::D3Q19>(f, rho, u);
+  else if (lat == LBM_D3Q27) st == LBM_F64 ? rhou_t<sch::Float64, sch::D3Q27>(f, rho, u) : rhou_t<sch::Float32, sch::D3Q27>(f, rho, u);
+  else return -1;
+  return 0;
+}
+
+double ref_porosity(int D, int st, const double* rel, double r, double eps) {
+  if (D == 2) return st == LBM_F64 ? porosity_t<sch::Float64, 2>(rel, r, eps) : porosity_t<sch::Float32, 2>(rel, r, eps);
+  return st == LBM_F64 ? porosity_t<sch::Float64, 3>(rel, r, eps) : porosity_t<sch::Float32, 3>(rel, r, eps);
+}
+
+int ref_aabb(int st, const lbm_particle* p, const uint64_t* extent, uint64_t* a, uint64_t* b) {
+  auto run = [&](auto zero) {
+    using T = decltype(zero);
+    using PG = sch::ParticleGroup<T, 2, 1, kl::coll::Spheroid<T>>;
+    auto g = make_group<T, 2, 1>(p);
+    kl::data<sch::FixedArray<sch::UInt64, 2>> meta{{extent[0], extent[1]}};
+    auto eo = kl::particle_aabb<PG>::calculate(g, {{0u}}, meta);
+    if (!eo.is_value()) return -1;
+    auto& v = eo.get_value();
+    for (uint64_t d = 0; d < 2; ++d) {
+      a[d] = v.template get<"a">().at({d}).get();
+      b[d] = v.template get<"b">().at({d}).get();
+    }
+    return 0;
+  };
+  return st == LBM_F64 ? run(sch::Float64{}) : run(sch::Float32{});
+}
+
+int ref_spheroid_mass(int D, int st, double r, double rho, double* m, double* I) {
+  lbm_particle p{};
+  p.radius = r;
+  p.density = rho;
+  auto run = [&](auto zero, auto dims) {
+    using T = decltype(zero);
+    constexpr uint64_t DD = decltype(dims)::value;
+    kl::data<sch::LbmMeter<sch::Scalar<T>>> rr;
+    kl::data<sch::Scalar<T>> dens;
+    rr.at({}).set(r);
+    dens.at({}).set(rho);
+    auto g = kl::create_spheroid_particle_group<T, DD, 1>(rr, dens, {4u});
+    auto& rb = g.template get<"particles">().at({0u}).template get<"rigid_body">();
+    *m = rb.template get<"mass">().at({}).get();
+    *I = rb.template get<"inertia">().at({}).get();
+  };
+  using D2 = std::integral_constant<uint64_t, 2>;
+  using D3 = std::integral_constant<uint64_t, 3>;
+  if (D == 2) st == LBM_F64 ? run(sch::Float64{}, D2{}) : run(sch::Float32{}, D2{});
+  else st == LBM_F64 ? run(sch::Float64{}, D3{}) : run(sch::Float32{}, D3{});
+  return 0;
+}
+
+int ref_dem(int D, int st, uint32_t n, lbm_particle* p, const lbm_particle_params* prm,
+            const uint64_t* extent, int reset, int contact, int integrate) {
+  if (D == 2) return st == LBM_F64 ? dem_n<sch::Float64, 2>(n, p, prm, extent, reset, contact, integrate)
+                                   : dem_n<sch::Float32, 2>(n, p, prm, extent, reset, contact, integrate);
+  return st == LBM_F64 ? dem_n<sch::Float64, 3>(n, p, prm, extent, reset, contact, integrate)
+                       : dem_n<sch::Float32, 3>(n, p, prm, extent, reset, contact, integrate);
+}
+
+int64_t ref_couple(int st, uint32_t n, const lbm_particle* p, const uint64_t* extent,
+                   double max_radius, double eps, double* out) {
+  auto run = [&](auto zero) -> int64_t {
+    using T = decltype(zero);
+    switch (n) {
+      case 1: return int64_t(couple_t<T, 1>(p, extent, max_radius, eps, out));
+      case 2: return int64_t(couple_t<T, 2>(p, extent, max_radius, eps, out));
+      case 3: return int64_t(couple_t<T, 3>(p, extent, max_radius, eps, out));
+      case 4: return int64_t(couple_t<T, 4>(p, extent, max_radius, eps, out));
+    }
+    return -1;
+  };
+  return st == LBM_F64 ? run(sch::Float64{}) : run(sch::Float32{});
+}
+
+}  // extern "C"

@@ -220,9 +220,16 @@ def test_psm_couple_single_pass(hip, orc, st, mode):
         tol = (1e-12 if st == abi.F64 else 1e-5)
         assert np.all(np.abs(Fg - Fo) <= tol * A[:, None] + 1e-300)
         assert np.all(np.abs(Tg - To) <= tol * A[:, None] * 64 + 1e-300)
-        assert np.array_equal(g.get_porosity(), o.get_porosity())
+        # the porosity ramp calls sin (porosity.hpp:82): the device libm and
+        # glibc may differ by an ulp, so porosity agrees to 2 ulp of 1, and
+        # the per-cell force mom·(1−B) to that times the momentum scale
+        pg, po = g.get_porosity(), o.get_porosity()
+        ptol = 4.5e-16 if st == abi.F64 else 2.5e-7
+        assert np.max(np.abs(pg - po)) <= ptol
+        assert np.sum(pg < 1) == np.sum(po < 1) and np.sum(pg < 1) > 0
         ffg, ffo = g.get_force_field(), o.get_force_field()
-        assert np.array_equal(ffg, ffo)
+        fscale = max(np.max(np.abs(ffo)), 1e-300)
+        assert np.max(np.abs(ffg - ffo)) <= (1e-13 if st == abi.F64 else 1e-5) * fscale
         for s in (g, o):
             s.step(1) if k == 0 else None
         if k == 0:
@@ -258,8 +265,27 @@ def test_particle_laden_run(hip, orc, st):
             assert abs(a.position[d] - b.position[d]) <= tol * scale
     rg, ug = g.get_macros()
     ro, uo = o.get_macros()
-    assert np.max(np.abs(rg - ro)) <= (1e-12 if st == abi.F64 else 1e-5)
-    assert np.max(np.abs(ug - uo)) <= (1e-12 if st == abi.F64 else 1e-5) * max(np.max(np.abs(uo)), 1e-3)
+    if st == abi.F64:
+        assert np.max(np.abs(rg - ro)) <= 1e-12
+        assert np.max(np.abs(ug - uo)) <= 1e-12 * np.max(np.abs(uo))
+        return
+    # FP32: the particle force sums are FP64 tree sums on the GPU and serial
+    # FP32 sums in the reference, so the two FP32 runs differ by the
+    # reference's own FP32 summation error (~1e-5 of the velocity).  Bar:
+    # against the FP64 oracle the GPU's FP32 error is no larger than the
+    # reference arithmetic's FP32 error, and the two FP32 runs agree to 1e-4.
+    o64 = orc.Sim(abi.D2Q9, abi.F64, ext)
+    _setup((o64,), flags, ops)
+    o64.particles_set(parts, prm)
+    o64.step(60)
+    r64, u64 = o64.get_macros()
+    us = np.max(np.abs(u64))
+    err_gpu = np.max(np.abs(ug - u64)) / us
+    err_ref32 = np.max(np.abs(uo - u64)) / us
+    print(f"FP32 particle-laden: GPU err vs FP64 {err_gpu:.2e}, reference-arithmetic FP32 err {err_ref32:.2e}")
+    assert err_gpu <= 2.0 * err_ref32 + 1e-6
+    assert np.max(np.abs(ug - uo)) <= 1e-4 * np.max(np.abs(uo))
+    assert np.max(np.abs(rg - ro)) <= 1e-5
 
 
 def test_dem_contact_and_integrate_bitwise(hip, orc):

@@ -620,6 +620,13 @@ def test_poiseuille_profile_shape(orc):
     scale = ux[inner].sum() / prof[inner].sum()
     assert np.max(np.abs(ux[inner] - scale * prof[inner])) < 2e-2 * scale
     assert 0.02 < scale < 0.05  # flux below nominal, as the reference inlet gives
+    # inlet cells: after ZouHeVelocityX<true> ρ = ρ_w(1+2u), j_x = ρ_w u, so
+    # the macro velocity written by the BGK that follows is u/(1+2u) (Q2)
+    u_in = parabolic_inlet(nx, ny, 0.05)[0].reshape(ny, nx)[1:-1, 0]
+    ux_in = u[0].reshape(ny, nx)[1:-1, 0]
+    uy_in = u[1].reshape(ny, nx)[1:-1, 0]
+    assert np.allclose(ux_in, u_in / (1 + 2 * u_in), rtol=1e-13, atol=0)
+    assert np.max(np.abs(uy_in)) < 1e-15
 
 
 def test_mass_conservation_closed_box(orc):
