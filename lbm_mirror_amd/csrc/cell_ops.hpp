@@ -25,6 +25,8 @@ struct OpDev {
   double vec[3];
   double rho_eq;
   double disc[5];
+  uint32_t force_mask;  // BGKGuo: bit d set when force component d may be non-zero
+  uint32_t pad;
 };
 
 // compute_rho_u (macroscopic.hpp:10-35)
@@ -84,45 +86,93 @@ __device__ __forceinline__ void equilibrium(T rho, const T (&u)[3], T (&eq)[L::Q
   for (int i = 0; i < L::Q; ++i) eq[i] = eq_dir<L, T, WB>(i, rho, u, vv);
 }
 
+// The collisions below visit the directions in opposite pairs (i, opp(i)).
+// For a pair, c·u of the opposite is the exact negation (IEEE rounding is
+// symmetric), so a, a², w·ρ and their double conversions are computed once
+// and both equilibria come from the same terms — each value is bitwise what
+// the per-direction evaluation gives.
+template <class L>
+__device__ __forceinline__ constexpr bool pair_head(int i) {
+  return i == 0 || i < L::opp[i];
+}
+
+template <class L, class T>
+__device__ __forceinline__ T dot_c(int i, const T (&u)[3]) {
+  T vc = T(0);
+#pragma unroll
+  for (int d = 0; d < L::D; ++d) {
+    const int c = cdir<L>(i, d);
+    if (c == 1) vc = vc + u[d];
+    if (c == -1) vc = vc - u[d];
+  }
+  return vc;
+}
+
+// eq_i and eq_opp(i) from one a = 3 c_i·u
+template <class L, class T, bool WB>
+__device__ __forceinline__ void eq_pair(int i, T rho, T a, T vv, T& ei, T& eo) {
+  const T wr = T(L::w[i]) * rho;
+  const T aa = a * a;
+  if (WB) {
+    const double A = double(a), C = 1.5 * double(vv), Y = double(aa) * 0.5, W = double(wr);
+    ei = T(W * (((1.0 + A) - C) + Y));
+    eo = T(W * (((1.0 - A) - C) + Y));
+  } else {
+    const T C = T(1.5) * vv, Y = aa * T(0.5);
+    ei = wr * (((T(1) + a) - C) + Y);
+    eo = wr * (((T(1) - a) - C) + Y);
+  }
+}
+
 // BGK relaxation (collision.hpp:91-95)
 template <class L, class T, bool WB>
 __device__ __forceinline__ void bgk(T (&f)[L::Q], T rho, const T (&u)[3], T om) {
   const T vv = vel_sq<L, T>(u);
 #pragma unroll
   for (int i = 0; i < L::Q; ++i) {
-    const T e = eq_dir<L, T, WB>(i, rho, u, vv);
-    f[i] = f[i] + om * (e - f[i]);
+    if (!pair_head<L>(i)) continue;
+    const int o = L::opp[i];
+    const T a = dot_c<L, T>(i, u) * T(3.0);
+    T ei, eo;
+    eq_pair<L, T, WB>(i, rho, a, vv, ei, eo);
+    f[i] = f[i] + om * (ei - f[i]);
+    if (o != i) f[o] = f[o] + om * (eo - f[o]);
   }
 }
 
-// BGK with Guo source (collision.hpp:280-315), F = cell force + global force
+// BGK with Guo source (collision.hpp:280-315), F = cell force + global force.
+// Components outside `fmask` are exactly zero, and (t1+t2)·0 adds nothing to
+// the projection, so they are skipped.
 template <class L, class T, bool WB>
 __device__ __forceinline__ void bgk_guo(T (&f)[L::Q], T rho, const T (&u)[3], T om,
-                                        const T (&F)[3]) {
+                                        const T (&F)[3], uint32_t fmask) {
   const T vv = vel_sq<L, T>(u);
   const T relax = T(1) - T(0.5f) * om;
   const T ics2 = T(3.0);
+  T u3[3];
+#pragma unroll
+  for (int d = 0; d < L::D; ++d) u3[d] = u[d];
 #pragma unroll
   for (int i = 0; i < L::Q; ++i) {
-    const T e = eq_dir<L, T, WB>(i, rho, u, vv);
-    T cu = T(0);
+    if (!pair_head<L>(i)) continue;
+    const int o = L::opp[i];
+    const T cu = dot_c<L, T>(i, u);
+    const T a = cu * ics2;
+    T ei, eo;
+    eq_pair<L, T, WB>(i, rho, a, vv, ei, eo);
+    const T s2 = a * ics2;  // (c·u·3)·3; the opposite's is −s2, and c·s2 is shared
+    T pi = T(0), po = T(0);
 #pragma unroll
     for (int d = 0; d < L::D; ++d) {
-      const int c = cdir<L>(i, d);
-      if (c == 1) cu = cu + u[d];
-      if (c == -1) cu = cu - u[d];
-    }
-    const T s2 = (cu * ics2) * ics2;
-    T proj = T(0);
-#pragma unroll
-    for (int d = 0; d < L::D; ++d) {
+      if (!(fmask & (1u << d))) continue;
       const T cd = T(cdir<L>(i, d));
-      const T t1 = (cd - u[d]) * ics2;
       const T t2 = cd * s2;
-      proj = proj + (t1 + t2) * F[d];
+      pi = pi + ((cd - u3[d]) * ics2 + t2) * F[d];
+      po = po + (((-cd) - u3[d]) * ics2 + t2) * F[d];
     }
-    const T Fi = T(L::w[i]) * proj;
-    f[i] = (f[i] + om * (e - f[i])) + Fi * relax;
+    const T wi = T(L::w[i]);
+    f[i] = (f[i] + om * (ei - f[i])) + (wi * pi) * relax;
+    if (o != i) f[o] = (f[o] + om * (eo - f[o])) + (wi * po) * relax;
   }
 }
 
@@ -136,9 +186,12 @@ __device__ __forceinline__ void psm(T (&f)[L::Q], T rho, const T (&u)[3], T om, 
   for (int i = 0; i < L::Q; ++i) cpy[i] = f[i];
 #pragma unroll
   for (int i = 0; i < L::Q; ++i) {
-    const T e = eq_dir<L, T, WB>(i, rho, u, vv);
+    if (!pair_head<L>(i)) continue;
     const int o = L::opp[i];
-    f[i] = (cpy[i] + (om * (e - cpy[i])) * B) + (cpy[o] - cpy[i]) * flip;
+    T ei, eo;
+    eq_pair<L, T, WB>(i, rho, dot_c<L, T>(i, u) * T(3.0), vv, ei, eo);
+    f[i] = (cpy[i] + (om * (ei - cpy[i])) * B) + (cpy[o] - cpy[i]) * flip;
+    if (o != i) f[o] = (cpy[o] + (om * (eo - cpy[o])) * B) + (cpy[i] - cpy[o]) * flip;
   }
 }
 

@@ -29,7 +29,8 @@ struct StepArgs {
 };
 
 struct LaunchShape {
-  int bx = 256, by = 1;  // threads per block along x and y
+  int bx = 256, by = 1;  // threads per block along x and y (scalar kernel)
+  bool vec = true;       // vectorised kernel where the row length allows it
 };
 
 hipError_t launch_aa_step(int lattice, int storage, bool odd, const StepArgs& a,

@@ -305,16 +305,16 @@ lbm_status lbm_ctx_create(lbm_ctx** out, int32_t lattice, int32_t storage,
 
 lbm_status lbm_ctx_destroy(lbm_ctx* c) {
   if (!c) return LBM_OK;
-  hipSetDevice(c->device);
-  if (c->stream) hipStreamSynchronize(c->stream);
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
   void* ptrs[] = {c->f, c->flags, c->ops, c->rho, c->u, c->force, c->por, c->bcvel,
                   c->parts, c->f64, c->t64, c->partial, c->cl_count, c->cl_slot,
                   c->cl_overflow, c->owner};
   for (void* p : ptrs)
-    if (p) hipFree(p);
-  if (c->ev0) hipEventDestroy(c->ev0);
-  if (c->ev1) hipEventDestroy(c->ev1);
-  if (c->stream) hipStreamDestroy(c->stream);
+    if (p) (void)hipFree(p);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return LBM_OK;
 }
@@ -376,6 +376,10 @@ lbm_status lbm_set_operator(lbm_ctx* c, uint8_t flag, const lbm_op_desc* op) {
   for (int d = 0; d < 3; ++d) o.vec[d] = to_T(st, op->vec[d]);
   o.rho_eq = to_T(st, op->rho_eq);
   for (int k = 0; k < 5; ++k) o.disc[k] = to_T(st, op->disc[k]);
+  // components of F = cell force + global force that can be non-zero
+  o.force_mask = 0;
+  for (int d = 0; d < c->D; ++d)
+    if ((o.flags & LBM_OPF_GUO_CELL_FORCE) || o.vec[d] != 0.0) o.force_mask |= 1u << d;
   if (o.coll == LBM_COLL_PSM) {
     lbm_status s = ensure_field(c, &c->por, c->N, 1.0);
     if (s) return s;

@@ -23,6 +23,29 @@
 
 namespace lbmx {
 
+// Build-time tuning of the fused kernel: vector width for FP32 (4, 2, or 0 =
+// scalar kernel), whether FP64 and the odd (shifted) phase use it.
+#ifndef LBMX_VEC_F32
+#define LBMX_VEC_F32 0
+#endif
+#ifndef LBMX_VEC_F64
+#define LBMX_VEC_F64 0
+#endif
+#ifndef LBMX_VEC_ODD
+#define LBMX_VEC_ODD 1
+#endif
+#ifndef LBMX_ROW_MIN_NX
+#define LBMX_ROW_MIN_NX 64
+#endif
+constexpr int kVecF32 = LBMX_VEC_F32;
+constexpr int kRowKernelMinNx = LBMX_ROW_MIN_NX;
+#ifndef LBMX_ROW_BLOCK
+#define LBMX_ROW_BLOCK 256
+#endif
+constexpr int kRowBlock = LBMX_ROW_BLOCK;
+constexpr bool kVecF64 = LBMX_VEC_F64 != 0;
+constexpr bool kVecOdd = LBMX_VEC_ODD != 0;
+
 // ---- storage traits --------------------------------------------------------
 template <class S, class T>
 struct Store;
@@ -32,10 +55,19 @@ struct Store<double, double> {
   __device__ static void st(double* p, double v, double) { *p = v; }
   __device__ static double zero(double) { return 0.0; }
 };
+#ifndef LBMX_NT
+#define LBMX_NT 0
+#endif
 template <>
 struct Store<float, float> {
-  __device__ static float ld(const float* p, float) { return *p; }
-  __device__ static void st(float* p, float v, float) { *p = v; }
+  __device__ static float ld(const float* p, float) {
+    if (LBMX_NT & 1) return __builtin_nontemporal_load(p);
+    return *p;
+  }
+  __device__ static void st(float* p, float v, float) {
+    if (LBMX_NT & 2) __builtin_nontemporal_store(v, p);
+    else *p = v;
+  }
 };
 // Shifted 16-bit storage: the stored value is f_i − w_i.
 template <>
@@ -51,51 +83,14 @@ struct Store<__hip_bfloat16, float> {
   __device__ static void st(__hip_bfloat16* p, float v, float w) { *p = __float2bfloat16(v - w); }
 };
 
-template <class L, class S, class T, bool WB, bool ODD>
-__global__ __launch_bounds__(256) void k_aa_step(StepArgs a) {
+// Boundary then collision for one cell (the drivers' switch(info)), writing
+// the macros when asked.  f holds the pre-collision populations on entry and
+// the post-collision ones on exit.
+template <class L, class T, bool WB>
+__device__ __forceinline__ void apply_op(T (&f)[L::Q], const OpDev* op, const StepArgs& a,
+                                         uint32_t cell, int x, int y, int z) {
   constexpr int Q = L::Q;
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  const int y = blockIdx.y * blockDim.y + threadIdx.y;
-  const int z = blockIdx.z;
-  if (x >= a.nx || y >= a.ny) return;
-  const int nx = a.nx, ny = a.ny, nz = a.nz;
-  const uint32_t sxy = uint32_t(nx) * uint32_t(ny);
-  const uint32_t cell = uint32_t(x) + uint32_t(nx) * (uint32_t(y) + uint32_t(ny) * uint32_t(z));
-  S* __restrict__ fb = static_cast<S*>(a.f);
-  const uint32_t Np = a.Np;
-
-  // neighbour coordinates (wrapped) and validity per axis, for offsets −1, 0, +1
-  const bool px = a.periodic & 1u, py = a.periodic & 2u, pz = a.periodic & 4u;
-  const uint32_t X[3] = {uint32_t(x == 0 ? nx - 1 : x - 1), uint32_t(x),
-                         uint32_t(x == nx - 1 ? 0 : x + 1)};
-  const uint32_t Y[3] = {uint32_t(y == 0 ? ny - 1 : y - 1) * uint32_t(nx), uint32_t(y) * uint32_t(nx),
-                         uint32_t(y == ny - 1 ? 0 : y + 1) * uint32_t(nx)};
-  const uint32_t Z[3] = {uint32_t(z == 0 ? nz - 1 : z - 1) * sxy, uint32_t(z) * sxy,
-                         uint32_t(z == nz - 1 ? 0 : z + 1) * sxy};
-  const bool VX[3] = {x > 0 || px, true, x < nx - 1 || px};
-  const bool VY[3] = {y > 0 || py, true, y < ny - 1 || py};
-  const bool VZ[3] = {L::D < 3 || z > 0 || pz, true, L::D < 3 || z < nz - 1 || pz};
-
-  T f[Q];
-  if (!ODD) {
-#pragma unroll
-    for (int i = 0; i < Q; ++i) f[i] = Store<S, T>::ld(fb + size_t(i) * Np + cell, T(L::w[i]));
-  } else {
-#pragma unroll
-    for (int i = 0; i < Q; ++i) {
-      const int ix = 1 - L::cx[i], iy = 1 - L::cy[i], iz = 1 - L::cz[i];  // x − c_i
-      if (VX[ix] && VY[iy] && VZ[iz]) {
-        const uint32_t src = X[ix] + Y[iy] + Z[iz];
-        f[i] = Store<S, T>::ld(fb + size_t(L::opp[i]) * Np + src, T(L::w[i]));
-      } else {
-        f[i] = T(0);
-      }
-    }
-  }
-
-  const uint8_t fl = a.flags[cell];
-  if (fl < kMaxOps) {
-    const OpDev* op = a.ops + fl;
+  {
     const int bc = op->bc;
     const int coll = op->coll;
     switch (bc) {
@@ -174,7 +169,7 @@ __global__ __launch_bounds__(256) void k_aa_step(StepArgs a) {
         const bool use_cf = op->flags & LBM_OPF_GUO_CELL_FORCE;
 #pragma unroll
         for (int d = 0; d < L::D; ++d) F[d] = (use_cf ? cf[size_t(d) * a.N + cell] : T(0)) + T(op->vec[d]);
-        bgk_guo<L, T, WB>(f, rho, u, om, F);
+        bgk_guo<L, T, WB>(f, rho, u, om, F, op->force_mask);
       } else if (coll == LBM_COLL_PSM) {
         psm<L, T, WB>(f, rho, u, om, B);
         if (op->flags & (LBM_OPF_PSM_CELL_FORCE | LBM_OPF_PSM_FORCE_GATHER)) {
@@ -188,6 +183,65 @@ __global__ __launch_bounds__(256) void k_aa_step(StepArgs a) {
       }
     }
   }
+}
+
+template <class L, class T, bool WB>
+__device__ __forceinline__ void collide_cell(T (&f)[L::Q], uint8_t fl, const StepArgs& a,
+                                             uint32_t cell, int x, int y, int z) {
+  // When every active lane of the wavefront carries the same flag (the bulk
+  // of any lattice), the operator is read with scalar loads and its kind is
+  // a scalar branch; mixed wavefronts take the per-lane path.
+  const int fl0 = __builtin_amdgcn_readfirstlane(int(fl));
+  if (__all(int(fl) == fl0)) {
+    if (fl0 < kMaxOps) apply_op<L, T, WB>(f, a.ops + fl0, a, cell, x, y, z);
+  } else {
+    if (fl < kMaxOps) apply_op<L, T, WB>(f, a.ops + fl, a, cell, x, y, z);
+  }
+}
+
+template <class L, class S, class T, bool WB, bool ODD>
+__global__ __launch_bounds__(256) void k_aa_step(StepArgs a) {
+  constexpr int Q = L::Q;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y * blockDim.y + threadIdx.y;
+  const int z = blockIdx.z;
+  if (x >= a.nx || y >= a.ny) return;
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const uint32_t sxy = uint32_t(nx) * uint32_t(ny);
+  const uint32_t cell = uint32_t(x) + uint32_t(nx) * (uint32_t(y) + uint32_t(ny) * uint32_t(z));
+  S* __restrict__ fb = static_cast<S*>(a.f);
+  const uint32_t Np = a.Np;
+
+  // neighbour coordinates (wrapped) and validity per axis, for offsets −1, 0, +1
+  const bool px = a.periodic & 1u, py = a.periodic & 2u, pz = a.periodic & 4u;
+  const uint32_t X[3] = {uint32_t(x == 0 ? nx - 1 : x - 1), uint32_t(x),
+                         uint32_t(x == nx - 1 ? 0 : x + 1)};
+  const uint32_t Y[3] = {uint32_t(y == 0 ? ny - 1 : y - 1) * uint32_t(nx), uint32_t(y) * uint32_t(nx),
+                         uint32_t(y == ny - 1 ? 0 : y + 1) * uint32_t(nx)};
+  const uint32_t Z[3] = {uint32_t(z == 0 ? nz - 1 : z - 1) * sxy, uint32_t(z) * sxy,
+                         uint32_t(z == nz - 1 ? 0 : z + 1) * sxy};
+  const bool VX[3] = {x > 0 || px, true, x < nx - 1 || px};
+  const bool VY[3] = {y > 0 || py, true, y < ny - 1 || py};
+  const bool VZ[3] = {L::D < 3 || z > 0 || pz, true, L::D < 3 || z < nz - 1 || pz};
+
+  T f[Q];
+  if (!ODD) {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) f[i] = Store<S, T>::ld(fb + size_t(i) * Np + cell, T(L::w[i]));
+  } else {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const int ix = 1 - L::cx[i], iy = 1 - L::cy[i], iz = 1 - L::cz[i];  // x − c_i
+      if (VX[ix] && VY[iy] && VZ[iz]) {
+        const uint32_t src = X[ix] + Y[iy] + Z[iz];
+        f[i] = Store<S, T>::ld(fb + size_t(L::opp[i]) * Np + src, T(L::w[i]));
+      } else {
+        f[i] = T(0);
+      }
+    }
+  }
+
+  collide_cell<L, T, WB>(f, a.flags[cell], a, cell, x, y, z);
 
   if (!ODD) {
 #pragma unroll
@@ -207,8 +261,324 @@ __global__ __launch_bounds__(256) void k_aa_step(StepArgs a) {
   }
 }
 
+
+// Row-block variant of the scalar kernel: a block covers a contiguous run of
+// one row (blockIdx.y = y, blockIdx.z = z), so the row offsets of the ±1
+// neighbour rows and their validity are wavefront-uniform, and the partial
+// cache lines that the ±1 shifts along x create stay inside one block.
+template <class L, class S, class T, bool WB, bool ODD>
+__global__ __launch_bounds__(1024) void k_aa_row(StepArgs a) {
+  constexpr int Q = L::Q;
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y, z = blockIdx.z;
+  if (x >= nx) return;
+  const uint32_t sxy = uint32_t(nx) * uint32_t(ny);
+  const bool px = a.periodic & 1u, py = a.periodic & 2u, pz = a.periodic & 4u;
+  // wavefront-uniform: the base of every (plane, neighbour row) pair, and
+  // whether that row exists
+  const uint32_t Y[3] = {uint32_t(y == 0 ? ny - 1 : y - 1) * uint32_t(nx), uint32_t(y) * uint32_t(nx),
+                         uint32_t(y == ny - 1 ? 0 : y + 1) * uint32_t(nx)};
+  const uint32_t Z[3] = {uint32_t(z == 0 ? nz - 1 : z - 1) * sxy, uint32_t(z) * sxy,
+                         uint32_t(z == nz - 1 ? 0 : z + 1) * sxy};
+  const bool VY[3] = {y > 0 || py, true, y < ny - 1 || py};
+  const bool VZ[3] = {L::D < 3 || z > 0 || pz, true, L::D < 3 || z < nz - 1 || pz};
+  // per lane: x neighbours, as 32-bit offsets within a row
+  const uint32_t X[3] = {uint32_t(x == 0 ? nx - 1 : x - 1), uint32_t(x),
+                         uint32_t(x == nx - 1 ? 0 : x + 1)};
+  const bool VX[3] = {x > 0 || px, true, x < nx - 1 || px};
+  const uint32_t row = Y[1] + Z[1];
+  S* __restrict__ fb = static_cast<S*>(a.f);
+  const size_t Np = a.Np;
+  auto rowp = [&](int plane, int iy, int iz) -> S* {
+    return fb + (size_t(plane) * Np + (Y[iy] + Z[iz]));
+  };
+  // every neighbour of every lane in the lattice (or wrapped): no predication
+  const bool inner_rows = VY[0] && VY[2] && VZ[0] && VZ[2];
+  const bool inner = inner_rows && __all(VX[0] && VX[2]);
+
+  T f[Q];
+  if (!ODD) {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) f[i] = Store<S, T>::ld(rowp(i, 1, 1) + X[1], T(L::w[i]));
+  } else if (inner) {
+#pragma unroll
+    for (int i = 0; i < Q; ++i)
+      f[i] = Store<S, T>::ld(rowp(L::opp[i], 1 - L::cy[i], 1 - L::cz[i]) + X[1 - L::cx[i]], T(L::w[i]));
+  } else {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const int ix = 1 - L::cx[i], iy = 1 - L::cy[i], iz = 1 - L::cz[i];  // x − c_i
+      f[i] = (VX[ix] && VY[iy] && VZ[iz]) ? Store<S, T>::ld(rowp(L::opp[i], iy, iz) + X[ix], T(L::w[i]))
+                                          : T(0);
+    }
+  }
+
+  collide_cell<L, T, WB>(f, a.flags[row + X[1]], a, row + X[1], x, y, z);
+
+  if (!ODD) {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) Store<S, T>::st(rowp(L::opp[i], 1, 1) + X[1], f[i], T(L::w[i]));
+  } else if (inner) {
+#pragma unroll
+    for (int i = 0; i < Q; ++i)
+      Store<S, T>::st(rowp(i, 1 + L::cy[i], 1 + L::cz[i]) + X[1 + L::cx[i]], f[i], T(L::w[i]));
+  } else {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const int ix = 1 + L::cx[i], iy = 1 + L::cy[i], iz = 1 + L::cz[i];  // x + c_i
+      if (VX[ix] && VY[iy] && VZ[iz]) Store<S, T>::st(rowp(i, iy, iz) + X[ix], f[i], T(L::w[i]));
+      const int jx = 1 - L::cx[i], jy = 1 - L::cy[i], jz = 1 - L::cz[i];
+      if (!(VX[jx] && VY[jy] && VZ[jz])) Store<S, T>::st(rowp(i, 1, 1) + X[1], T(0), T(L::w[i]));
+    }
+  }
+}
+
+// ---- vectorised variant ------------------------------------------------------
+// Each thread owns V consecutive cells of one row and moves every plane with
+// one aligned V-wide access.  In phase 1 the ±1 shifts along x are resolved
+// inside the 64-wide wavefront: a lane takes the element it needs from its
+// neighbour lane's aligned load (__shfl_up/__shfl_down) and hands its edge
+// value to the neighbour for the shifted stores; only the first and last lane
+// of a row segment (row start/end, or wavefront boundary) issue one extra
+// scalar access.  Every plane element is still written exactly once.
+template <class S, int V>
+struct VecT;
+template <>
+struct VecT<float, 4> {
+  using type = float4;
+};
+template <>
+struct VecT<float, 2> {
+  using type = float2;
+};
+template <>
+struct VecT<double, 2> {
+  using type = double2;
+};
+
+template <class S, int V>
+__device__ __forceinline__ void vload(const S* p, S (&e)[V]) {
+  const typename VecT<S, V>::type t = *reinterpret_cast<const typename VecT<S, V>::type*>(p);
+  const S* q = reinterpret_cast<const S*>(&t);
+#pragma unroll
+  for (int v = 0; v < V; ++v) e[v] = q[v];
+}
+template <class S, int V>
+__device__ __forceinline__ void vstore(S* p, const S (&e)[V]) {
+  typename VecT<S, V>::type t;
+  S* q = reinterpret_cast<S*>(&t);
+#pragma unroll
+  for (int v = 0; v < V; ++v) q[v] = e[v];
+  *reinterpret_cast<typename VecT<S, V>::type*>(p) = t;
+}
+
+template <class L, class T, bool WB, bool ODD, int V>
+__global__ __launch_bounds__(256) void k_aa_vec(StepArgs a) {
+  constexpr int Q = L::Q;
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const int x0 = (blockIdx.x * blockDim.x + threadIdx.x) * V;
+  const int yt = blockIdx.y * blockDim.y + threadIdx.y;
+  const int z = blockIdx.z;
+  const bool active = x0 < nx && yt < ny;
+  const int y = active ? yt : 0;
+  const int lane = __lane_id();
+  const bool seg_start = x0 == 0 || lane == 0;
+  const bool seg_end = x0 + V >= nx || lane == 63;
+  const uint32_t sxy = uint32_t(nx) * uint32_t(ny);
+  const bool px = a.periodic & 1u, py = a.periodic & 2u, pz = a.periodic & 4u;
+  const uint32_t Y[3] = {uint32_t(y == 0 ? ny - 1 : y - 1) * uint32_t(nx), uint32_t(y) * uint32_t(nx),
+                         uint32_t(y == ny - 1 ? 0 : y + 1) * uint32_t(nx)};
+  const uint32_t Z[3] = {uint32_t(z == 0 ? nz - 1 : z - 1) * sxy, uint32_t(z) * sxy,
+                         uint32_t(z == nz - 1 ? 0 : z + 1) * sxy};
+  const bool VY[3] = {y > 0 || py, true, y < ny - 1 || py};
+  const bool VZ[3] = {L::D < 3 || z > 0 || pz, true, L::D < 3 || z < nz - 1 || pz};
+  const uint32_t row = Y[1] + Z[1];
+  T* __restrict__ fb = static_cast<T*>(a.f);
+  const size_t Np = a.Np;
+
+  T f[V][Q];
+  if (!ODD) {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      T e[V];
+      if (active) vload<T, V>(fb + size_t(i) * Np + row + x0, e);
+      else
+#pragma unroll
+        for (int v = 0; v < V; ++v) e[v] = T(0);
+#pragma unroll
+      for (int v = 0; v < V; ++v) f[v][i] = e[v];
+    }
+  } else {
+    // all aligned loads first, then the edge elements, then the shuffles, so
+    // that every load of the cell block is in flight at once
+    T e[Q][V];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const int cy = L::cy[i], cz = L::cz[i];
+      const bool src_ok = active && VY[1 - cy] && VZ[1 - cz];
+      const T* pl = fb + size_t(L::opp[i]) * Np + (Y[1 - cy] + Z[1 - cz]);
+      if (src_ok) vload<T, V>(pl + x0, e[i]);
+      else
+#pragma unroll
+        for (int v = 0; v < V; ++v) e[i][v] = T(0);
+    }
+    T edge[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const int cx = L::cx[i], cy = L::cy[i], cz = L::cz[i];
+      const bool src_ok = active && VY[1 - cy] && VZ[1 - cz];
+      const T* pl = fb + size_t(L::opp[i]) * Np + (Y[1 - cy] + Z[1 - cz]);
+      edge[i] = T(0);
+      if (cx == 1) {  // element x0-1 for the first lane of a segment
+        const bool inrow = x0 > 0;
+        if (seg_start && src_ok && (inrow || px)) edge[i] = pl[inrow ? x0 - 1 : nx - 1];
+      } else if (cx == -1) {  // element x0+V for the last lane of a segment
+        const bool inrow = x0 + V < nx;
+        if (seg_end && src_ok && (inrow || px)) edge[i] = pl[inrow ? x0 + V : 0];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const int cx = L::cx[i];
+      if (cx == 0) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) f[v][i] = e[i][v];
+      } else if (cx == 1) {  // needs x0-1 .. x0+V-2
+        const T up = __shfl_up(e[i][V - 1], 1, 64);
+        f[0][i] = seg_start ? edge[i] : up;
+#pragma unroll
+        for (int v = 1; v < V; ++v) f[v][i] = e[i][v - 1];
+      } else {  // needs x0+1 .. x0+V
+        const T dn = __shfl_down(e[i][0], 1, 64);
+#pragma unroll
+        for (int v = 0; v < V - 1; ++v) f[v][i] = e[i][v + 1];
+        f[V - 1][i] = seg_end ? edge[i] : dn;
+      }
+    }
+  }
+
+  if (active) {
+    uint8_t fl[V];
+    if (V == 4) {
+      const uchar4 q = *reinterpret_cast<const uchar4*>(a.flags + row + x0);
+      fl[0] = q.x;
+      fl[1 % V] = q.y;
+      fl[2 % V] = q.z;
+      fl[3 % V] = q.w;
+    } else {
+      const uchar2 q = *reinterpret_cast<const uchar2*>(a.flags + row + x0);
+      fl[0] = q.x;
+      fl[1 % V] = q.y;
+    }
+#pragma unroll
+    for (int v = 0; v < V; ++v) collide_cell<L, T, WB>(f[v], fl[v], a, row + x0 + v, x0 + v, y, z);
+  }
+
+  if (!ODD) {
+    if (active) {
+#pragma unroll
+      for (int i = 0; i < Q; ++i) {
+        T e[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) e[v] = f[v][i];
+        vstore<T, V>(fb + size_t(L::opp[i]) * Np + row + x0, e);
+      }
+    }
+  } else {
+    T nb[Q];  // neighbour lane's edge value per x-shifted plane
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const int cx = L::cx[i];
+      nb[i] = T(0);
+      if (cx == 1) nb[i] = __shfl_up(f[V - 1][i], 1, 64);
+      if (cx == -1) nb[i] = __shfl_down(f[0][i], 1, 64);
+    }
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const int cx = L::cx[i], cy = L::cy[i], cz = L::cz[i];
+      T* pl = fb + size_t(i) * Np;
+      // own row: its upstream row is a ghost row, nobody streams into it
+      if (active && !(VY[1 - cy] && VZ[1 - cz])) {
+        T zr[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) zr[v] = T(0);
+        vstore<T, V>(pl + row + x0, zr);
+      }
+      const bool dst_ok = active && VY[1 + cy] && VZ[1 + cz];
+      T* pd = pl + (Y[1 + cy] + Z[1 + cz]);
+      if (!dst_ok) continue;
+      if (cx == 0) {
+        T ev[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) ev[v] = f[v][i];
+        vstore<T, V>(pd + x0, ev);
+      } else if (cx == 1) {  // cell x0+v lands on x0+v+1
+        if (!seg_start) {
+          T ev[V];
+          ev[0] = nb[i];
+#pragma unroll
+          for (int v = 1; v < V; ++v) ev[v] = f[v - 1][i];
+          vstore<T, V>(pd + x0, ev);
+        } else {
+#pragma unroll
+          for (int v = 1; v < V; ++v) pd[x0 + v] = f[v - 1][i];
+          if (x0 == 0 && !px) pd[0] = T(0);  // upstream of x = 0 is the ghost column
+        }
+        if (seg_end) {
+          if (x0 + V < nx) pd[x0 + V] = f[V - 1][i];
+          else if (px) pd[0] = f[V - 1][i];
+        }
+      } else {  // cx == -1: cell x0+v lands on x0+v-1
+        if (!seg_end) {
+          T ev[V];
+#pragma unroll
+          for (int v = 0; v < V - 1; ++v) ev[v] = f[v + 1][i];
+          ev[V - 1] = nb[i];
+          vstore<T, V>(pd + x0, ev);
+        } else {
+#pragma unroll
+          for (int v = 0; v < V - 1; ++v) pd[x0 + v] = f[v + 1][i];
+          if (x0 + V >= nx && !px) pd[x0 + V - 1] = T(0);  // upstream is the ghost column
+        }
+        if (seg_start) {
+          if (x0 > 0) pd[x0 - 1] = f[0][i];
+          else if (px) pd[nx - 1] = f[0][i];
+        }
+      }
+    }
+  }
+}
+
+template <class L, class T, bool WB, int V>
+static hipError_t launch_vec(bool odd, const StepArgs& a, hipStream_t s) {
+  const int tx = (a.nx + V - 1) / V;  // threads per row
+  int bx = 1;
+  while (bx < tx && bx < 256) bx *= 2;
+  const int by = 256 / bx;
+  dim3 block(bx, by, 1);
+  dim3 grid((tx + bx - 1) / bx, (a.ny + by - 1) / by, a.nz);
+  if (odd)
+    hipLaunchKernelGGL((k_aa_vec<L, T, WB, true, V>), grid, block, 0, s, a);
+  else
+    hipLaunchKernelGGL((k_aa_vec<L, T, WB, false, V>), grid, block, 0, s, a);
+  return hipGetLastError();
+}
+
 template <class L, class S, class T, bool WB>
 static hipError_t launch_t(bool odd, const StepArgs& a, const LaunchShape& sh, hipStream_t s) {
+  if (a.nx >= kRowKernelMinNx) {
+    // whole rows per block (up to 1024 cells), rows along grid y and z
+    int bx = 64;
+    while (bx < a.nx && bx < kRowBlock) bx *= 2;
+    dim3 block(bx, 1, 1);
+    dim3 grid((a.nx + bx - 1) / bx, a.ny, a.nz);
+    if (odd)
+      hipLaunchKernelGGL((k_aa_row<L, S, T, WB, true>), grid, block, 0, s, a);
+    else
+      hipLaunchKernelGGL((k_aa_row<L, S, T, WB, false>), grid, block, 0, s, a);
+    return hipGetLastError();
+  }
   dim3 block(sh.bx, sh.by, 1);
   dim3 grid((a.nx + sh.bx - 1) / sh.bx, (a.ny + sh.by - 1) / sh.by, a.nz);
   if (odd)
@@ -222,8 +592,13 @@ template <class L>
 static hipError_t launch_l(int storage, bool odd, const StepArgs& a, const LaunchShape& sh,
                            hipStream_t s) {
   switch (storage) {
-    case LBM_F64: return launch_t<L, double, double, true>(odd, a, sh, s);
-    case LBM_F32: return launch_t<L, float, float, true>(odd, a, sh, s);
+    case LBM_F64:
+      if (sh.vec && kVecF64 && a.nx % 2 == 0) return launch_vec<L, double, true, 2>(odd, a, s);
+      return launch_t<L, double, double, true>(odd, a, sh, s);
+    case LBM_F32:
+      if (sh.vec && kVecF32 > 0 && (!odd || kVecOdd) && a.nx % kVecF32 == 0)
+        return launch_vec<L, float, true, (kVecF32 > 0 ? kVecF32 : 4)>(odd, a, s);
+      return launch_t<L, float, float, true>(odd, a, sh, s);
     case LBM_F16S: return launch_t<L, __half, float, false>(odd, a, sh, s);
     case LBM_BF16S: return launch_t<L, __hip_bfloat16, float, false>(odd, a, sh, s);
   }

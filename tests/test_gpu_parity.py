@@ -115,12 +115,19 @@ def test_d3q27_zouhe_pressure_channel(hip, orc, st):
 
 
 @pytest.mark.parametrize("lat,ext", [(abi.D2Q9, (21, 13)), (abi.D3Q19, (9, 7, 6)),
-                                     (abi.D3Q27, (8, 7, 6))])
+                                     (abi.D3Q27, (8, 7, 6)),
+                                     # vectorised kernel: 5 threads per row (partial
+                                     # wavefront), and 65 threads per row (a row over
+                                     # two wavefronts, one active lane in the second)
+                                     (abi.D2Q9, (20, 7)), (abi.D2Q9, (260, 5)),
+                                     (abi.D3Q19, (12, 5, 4)), (abi.D3Q27, (260, 3, 3))])
 @pytest.mark.parametrize("periodic", [0, 3])
 def test_random_flags_every_operator(hip, orc, lat, ext, periodic):
     """Random flag mixture over every operator the lattice supports, random
     populations, odd and even step counts: exercises both AA phases, ghost
-    reads/writes, and the unpacking of the swapped layout."""
+    reads/writes, the unpacking of the swapped layout, and both the scalar
+    kernel (row length not a multiple of the vector width) and the vectorised
+    kernel's wavefront-edge handling."""
     rng = np.random.default_rng(99 + lat)
     N = int(np.prod(ext))
     D, Q, c, w, _ = orc.lattice(lat)
@@ -138,9 +145,6 @@ def test_random_flags_every_operator(hip, orc, lat, ext, periodic):
         ops[9] = abi.op_desc(boundary=abi.BC_ZOUHE_VELOCITY_X, collision=abi.COLL_BGK, east=0,
                              tau=0.8)
         ops[10] = abi.op_desc(boundary=abi.BC_ANTI_BOUNCE_BACK)
-        ops[11] = abi.op_desc(collision=abi.COLL_PSM, tau=0.75,
-                              flags=abi.OPF_PSM_STAMP_DISC | abi.OPF_PSM_FORCE_GATHER,
-                              disc=(10.3, 6.1, 0.0, 4.0, 1.5))
     if lat == abi.D3Q27:
         ops[8] = abi.op_desc(boundary=abi.BC_ZOUHE_PRESSURE_X, collision=abi.COLL_BGK, east=0,
                              tau=0.8, rho=0.999)
@@ -315,8 +319,9 @@ def test_shifted_16bit_storage_error_vs_fp64(hip, orc, st):
     """FP16/BF16 storage of f − w with FP32 arithmetic: reported error against
     the FP64 oracle on the config-1 channel (no reference counterpart; the
     bound is the storage precision times the deviation scale)."""
+    from lbm_mirror_amd.lattice import Lattice
     nx, ny = 64, 18
-    g, _ = _pair(hip, orc, abi.D2Q9, st, (nx, ny))
+    g = Lattice(abi.D2Q9, st, (nx, ny))
     o = orc.Sim(abi.D2Q9, abi.F64, (nx, ny))
     _setup((g, o), channel_flags_2d(nx, ny), c1_ops(0.8, 0.05), bcu=parabolic_inlet(nx, ny, 0.05))
     g.step(800)
@@ -328,3 +333,41 @@ def test_shifted_16bit_storage_error_vs_fp64(hip, orc, st):
     print(f"storage {st}: max rel velocity error {err_u:.2e}, max density error {err_r:.2e}")
     bound = 2e-2 if st == abi.F16S else 8e-2
     assert err_u < bound and err_r < bound / 10
+
+
+@pytest.mark.parametrize("st", [abi.F64, abi.F32])
+def test_static_disc_psm_driver(hip, orc, st):
+    """The static-obstacle PSM driver (poiseulle_particles_2d_psm_gpu/sim.cpp:
+    207-295): OneParticleAt stamp -> Psm -> ForceGather on fluid cells, BGK
+    after Zou/He pressure at inlet/outlet, bounce-back walls and corner
+    flag 5.  The stamp evaluates sin (porosity.hpp:82), where the device libm
+    and glibc may differ by an ulp, so: porosity within 2 ulp, populations and
+    macros within 1e-12 (FP64) / 1e-5 (FP32) relative."""
+    nx, ny = 128, 40
+    fl = channel_flags_2d(nx, ny).reshape(ny, nx)
+    fl[0, 0] = fl[ny - 1, 0] = fl[0, nx - 1] = fl[ny - 1, nx - 1] = 5
+    ops = {1: abi.op_desc(boundary=abi.BC_BOUNCE_BACK),
+           2: abi.op_desc(collision=abi.COLL_PSM, tau=0.8,
+                          flags=abi.OPF_PSM_STAMP_DISC | abi.OPF_PSM_FORCE_GATHER,
+                          disc=(nx * 0.25, ny * 0.5, 0.0, ny * 0.1, 1.5)),
+           3: abi.op_desc(boundary=abi.BC_ZOUHE_PRESSURE_X, collision=abi.COLL_BGK, east=1,
+                          tau=0.8, ramp=(1.0, 0.0002 / 64, 1.0002, 64)),
+           4: abi.op_desc(boundary=abi.BC_ZOUHE_PRESSURE_X, collision=abi.COLL_BGK, east=0,
+                          tau=0.8, rho=1.0),
+           5: abi.op_desc(boundary=abi.BC_BOUNCE_BACK)}
+    g, o = _pair(hip, orc, abi.D2Q9, st, (nx, ny))
+    _setup((g, o), fl.reshape(-1), ops)
+    g.step(301)
+    o.step(301)
+    pg, po = g.get_porosity(), o.get_porosity()
+    assert np.max(np.abs(pg - po)) <= (4.5e-16 if st == abi.F64 else 2.5e-7)
+    assert np.sum(pg < 1) > 0
+    tol = 1e-12 if st == abi.F64 else 1e-5
+    fg, fo = g.get_populations(), o.get_populations()
+    assert np.max(np.abs(fg - fo)) <= tol * np.max(np.abs(fo))
+    rg, ug = g.get_macros()
+    ro, uo = o.get_macros()
+    assert np.max(np.abs(rg - ro)) <= tol
+    assert np.max(np.abs(ug - uo)) <= tol * np.max(np.abs(uo))
+    ffg, ffo = g.get_force_field(), o.get_force_field()
+    assert np.max(np.abs(ffg - ffo)) <= tol * max(np.max(np.abs(ffo)), 1e-300)
