@@ -207,6 +207,41 @@ lbm_status lbm_particle_integrate(lbm_ctx* ctx);
 /* Cell-list overflow count of the last build (cell_list.hpp:96-101). */
 lbm_status lbm_cell_list_overflow(lbm_ctx* ctx, uint64_t* overflow);
 
+/* ---- batched ensembles (no reference counterpart) ------------------------
+ * `count` independent lattices of uneven size stepped by one persistent launch
+ * per step, chains of steps replayed from a hipGraph.  Each member is an
+ * ordinary lbm_ctx for set-up and read-back (flags, operators, populations,
+ * macros); step members only through the lbm_batch_* calls.               */
+typedef struct lbm_batch lbm_batch;
+
+typedef struct lbm_batch_tuning {
+  int32_t block;       /* threads per block = cells per tile row (64..256)  */
+  int32_t rows;        /* rows per tile                                     */
+  int32_t grid;        /* persistent blocks (0 = one per tile)              */
+  double ms_per_step;  /* measured by lbm_batch_autotune                    */
+} lbm_batch_tuning;
+
+/* extents: count*3 values (nz ignored for D2Q9) */
+lbm_status lbm_batch_create(lbm_batch** out, int32_t lattice, int32_t storage, uint32_t count,
+                            const uint64_t* extents, int32_t device);
+lbm_status lbm_batch_destroy(lbm_batch* b);
+/* The i-th member's context (owned by the batch; NULL if out of range). */
+lbm_ctx* lbm_batch_member(lbm_batch* b, uint32_t i);
+/* Advance every active member nsteps reference steps t0 .. t0+nsteps-1. */
+lbm_status lbm_batch_step(lbm_batch* b, uint64_t t0, uint64_t nsteps);
+/* Run to steady state: every check_every steps (even, >= 2) each active
+ * member's residual r = |u_t - u_{t-1}|_2 / |u_t|_2 over all cells is formed
+ * in FP64 on the GPU; members with r < tol stop.  Returns when all stopped or
+ * after max_steps.  steps_out[i]: steps member i ran; res_out[i]: its last r. */
+lbm_status lbm_batch_run(lbm_batch* b, uint64_t t0, double tol, uint64_t check_every,
+                         uint64_t max_steps, uint64_t* steps_out, double* res_out);
+/* Times candidate tile shapes on the batch (state restored afterwards) and
+ * keeps the fastest. */
+lbm_status lbm_batch_autotune(lbm_batch* b, lbm_batch_tuning* chosen);
+lbm_status lbm_batch_set_tuning(lbm_batch* b, const lbm_batch_tuning* tuning);
+/* Device time and member cell updates of the last step/run call. */
+lbm_status lbm_batch_stats(lbm_batch* b, double* ms, uint64_t* cell_updates);
+
 #ifdef __cplusplus
 }
 #endif

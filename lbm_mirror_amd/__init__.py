@@ -34,6 +34,8 @@ EXPORTS = [
     "lbm_set_macros", "lbm_step", "lbm_sync", "lbm_last_step_ms", "lbm_particles_set",
     "lbm_particles_get", "lbm_spheroid_mass", "lbm_psm_couple", "lbm_particle_reduce",
     "lbm_dem_contact", "lbm_particle_integrate", "lbm_cell_list_overflow",
+    "lbm_batch_create", "lbm_batch_destroy", "lbm_batch_member", "lbm_batch_step",
+    "lbm_batch_run", "lbm_batch_autotune", "lbm_batch_set_tuning", "lbm_batch_stats",
 ]
 
 
@@ -78,9 +80,19 @@ def load_library(path=None):
     L.lbm_dem_contact.argtypes = [vp]
     L.lbm_particle_integrate.argtypes = [vp]
     L.lbm_cell_list_overflow.argtypes = [vp, C.POINTER(C.c_uint64)]
+    u64p = C.POINTER(C.c_uint64)
+    L.lbm_batch_create.argtypes = [C.POINTER(vp), C.c_int32, C.c_int32, C.c_uint32, u64p, C.c_int32]
+    L.lbm_batch_destroy.argtypes = [vp]
+    L.lbm_batch_member.argtypes = [vp, C.c_uint32]
+    L.lbm_batch_member.restype = vp
+    L.lbm_batch_step.argtypes = [vp, C.c_uint64, C.c_uint64]
+    L.lbm_batch_run.argtypes = [vp, C.c_uint64, C.c_double, C.c_uint64, C.c_uint64, u64p, dp]
+    L.lbm_batch_autotune.argtypes = [vp, C.POINTER(abi.BatchTuning)]
+    L.lbm_batch_set_tuning.argtypes = [vp, C.POINTER(abi.BatchTuning)]
+    L.lbm_batch_stats.argtypes = [vp, dp, u64p]
     for name in EXPORTS:
         f = getattr(L, name)
-        if name not in ("lbm_last_error", "lbm_version", "lbm_device_count"):
+        if name not in ("lbm_last_error", "lbm_version", "lbm_device_count", "lbm_batch_member"):
             f.restype = st
     if path is None:
         _lib = L

@@ -127,3 +127,12 @@ def particle_params(k_n=1.0, zeta_n=0.3, gamma_t=0.0, mu=0.0, walls=False,
         p.body_acceleration[i] = body_acceleration[i] if i < len(body_acceleration) else 0.0
     p.eps, p.max_radius = eps, max_radius
     return p
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [
+        ("block", C.c_int32),
+        ("rows", C.c_int32),
+        ("grid", C.c_int32),
+        ("ms_per_step", C.c_double),
+    ]

@@ -36,6 +36,40 @@ struct LaunchShape {
 hipError_t launch_aa_step(int lattice, int storage, bool odd, const StepArgs& a,
                           const LaunchShape& shape, hipStream_t s);
 
+// ---- batched ensembles -------------------------------------------------------
+// A tile is `rows` consecutive rows (from y) of one member, blockDim cells wide
+// from x0.
+struct Tile {
+  uint32_t member, y, z, x0;
+};
+
+struct BatchArgs {
+  const StepArgs* members;  // device array; t and write_macros are set per launch
+  const Tile* tiles;
+  uint32_t ntiles;
+  uint32_t rows;            // rows per tile
+  const uint64_t* t0;       // device: step index of the first step of a chain
+  uint32_t k;               // this launch's offset in the chain
+  int32_t write_macros;     // 0 none, 1 into the members' velocity, 2 into u_prev
+  void* const* u_prev;      // per member: second velocity field for the residual
+};
+
+struct BatchShape {
+  int block = 256;          // threads per block (cells per tile row)
+  int rows = 1;             // rows per tile
+  int grid = 0;             // persistent blocks (0: one per tile)
+};
+
+hipError_t launch_aa_batch(int lattice, int storage, bool odd, const BatchArgs& b,
+                           const BatchShape& shape, hipStream_t s);
+
+// Per member: Σ|u − u_prev|² and Σ|u|² in FP64 over every cell, tile partials
+// reduced in a fixed order.  partial: 2 doubles per tile; out: 2 per member;
+// tile_begin: ntiles-of-member prefix (members+1 entries).
+hipError_t launch_batch_residual(int dims, int storage, const BatchArgs& b, const BatchShape& shape,
+                                 double* partial, const uint32_t* tile_begin, uint32_t members,
+                                 double* out, hipStream_t s);
+
 // Particle coupling (particles.hip)
 struct ParticleArgs {
   // lattice and fields

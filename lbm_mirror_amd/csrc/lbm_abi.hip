@@ -756,3 +756,421 @@ lbm_status lbm_cell_list_overflow(lbm_ctx* c, uint64_t* overflow) {
 }
 
 }  // extern "C"
+
+// ================================================================ batches
+struct lbm_batch {
+  int lattice = 0, storage = 0, D = 2, device = 0;
+  std::vector<lbm_ctx*> members;
+  std::vector<char> active;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0.0;
+  uint64_t last_updates = 0;
+  int phase = 0;
+  BatchShape shape;
+  // device work list
+  StepArgs* d_members = nullptr;
+  Tile* d_tiles = nullptr;
+  uint32_t* d_tile_begin = nullptr;
+  void** d_uprev = nullptr;
+  double* d_partial = nullptr;
+  double* d_res = nullptr;
+  uint64_t* d_t0 = nullptr;
+  uint64_t* h_t0 = nullptr;  // pinned staging for the chain's first step index
+  std::vector<void*> uprev;
+  uint32_t ntiles = 0;
+  size_t tiles_cap = 0;
+  bool tiles_dirty = true;
+  // captured chains: key = (length, with residual)
+  hipGraphExec_t chain = nullptr;
+  uint64_t chain_len = 0;
+  int chain_res = -1;
+};
+
+namespace {
+
+StepArgs member_args(const lbm_ctx* c) {
+  StepArgs a{};
+  a.f = c->f;
+  a.flags = c->flags;
+  a.ops = c->ops;
+  a.rho = c->rho;
+  a.u = c->u;
+  a.force = c->force;
+  a.por = c->por;
+  a.bcvel = c->bcvel;
+  a.nx = int32_t(c->n[0]);
+  a.ny = int32_t(c->n[1]);
+  a.nz = int32_t(c->n[2]);
+  a.N = c->N;
+  a.Np = c->Np;
+  a.periodic = c->periodic;
+  return a;
+}
+
+void drop_chain(lbm_batch* b) {
+  if (b->chain) (void)hipGraphExecDestroy(b->chain);
+  b->chain = nullptr;
+  b->chain_len = 0;
+  b->chain_res = -1;
+}
+
+// Tiles of the active members, member by member; members' argument records.
+lbm_status build_tiles(lbm_batch* b) {
+  const uint32_t M = uint32_t(b->members.size());
+  std::vector<StepArgs> args(M);
+  std::vector<Tile> tiles;
+  std::vector<uint32_t> begin(M + 1, 0);
+  const int bx = b->shape.block, rows = b->shape.rows;
+  for (uint32_t m = 0; m < M; ++m) {
+    lbm_ctx* c = b->members[m];
+    args[m] = member_args(c);
+    begin[m] = uint32_t(tiles.size());
+    if (!b->active[m]) continue;
+    for (uint64_t z = 0; z < c->n[2]; ++z)
+      for (uint64_t y = 0; y < c->n[1]; y += uint64_t(rows))
+        for (uint64_t x0 = 0; x0 < c->n[0]; x0 += uint64_t(bx))
+          tiles.push_back(Tile{m, uint32_t(y), uint32_t(z), uint32_t(x0)});
+  }
+  begin[M] = uint32_t(tiles.size());
+  HIP_TRY(hipSetDevice(b->device));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  if (tiles.size() > b->tiles_cap) {
+    if (b->d_tiles) HIP_TRY(hipFree(b->d_tiles));
+    if (b->d_partial) HIP_TRY(hipFree(b->d_partial));
+    b->tiles_cap = std::max<size_t>(tiles.size(), 1);
+    HIP_TRY(hipMalloc((void**)&b->d_tiles, b->tiles_cap * sizeof(Tile)));
+    HIP_TRY(hipMalloc((void**)&b->d_partial, b->tiles_cap * 2 * sizeof(double)));
+  }
+  b->ntiles = uint32_t(tiles.size());
+  if (!tiles.empty())
+    HIP_TRY(hipMemcpyAsync(b->d_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(hipMemcpyAsync(b->d_members, args.data(), M * sizeof(StepArgs), hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(hipMemcpyAsync(b->d_tile_begin, begin.data(), (M + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  b->tiles_dirty = false;
+  drop_chain(b);
+  return LBM_OK;
+}
+
+BatchArgs batch_args(lbm_batch* b, uint32_t k, int write_macros) {
+  BatchArgs a{};
+  a.members = b->d_members;
+  a.tiles = b->d_tiles;
+  a.ntiles = b->ntiles;
+  a.rows = uint32_t(b->shape.rows);
+  a.t0 = b->d_t0;
+  a.k = k;
+  a.write_macros = write_macros;
+  a.u_prev = b->d_uprev;
+  return a;
+}
+
+// Enqueue a chain of n steps from the phase the batch is in; with_res also
+// writes u_prev at step n-2, u at step n-1 and forms the residual sums.
+lbm_status enqueue_chain(lbm_batch* b, uint64_t n, bool with_res) {
+  int phase = b->phase;
+  for (uint64_t k = 0; k < n; ++k) {
+    int wm = 0;
+    if (k + 1 == n) wm = 1;
+    if (with_res && k + 2 == n) wm = 2;
+    HIP_TRY(launch_aa_batch(b->lattice, b->storage, phase == 1, batch_args(b, uint32_t(k), wm), b->shape, b->stream));
+    phase ^= 1;
+  }
+  if (with_res)
+    HIP_TRY(launch_batch_residual(b->D, b->storage, batch_args(b, 0, 0), b->shape, b->d_partial,
+                                  b->d_tile_begin, uint32_t(b->members.size()), b->d_res, b->stream));
+  return LBM_OK;
+}
+
+// Run n steps starting at step t0 through a captured graph (rebuilt when the
+// chain length, the residual flag, the phase parity or the work list change).
+lbm_status run_chain(lbm_batch* b, uint64_t t0, uint64_t n, bool with_res) {
+  if (b->tiles_dirty) {
+    lbm_status s = build_tiles(b);
+    if (s) return s;
+  }
+  *b->h_t0 = t0;
+  HIP_TRY(hipMemcpyAsync(b->d_t0, b->h_t0, sizeof(uint64_t), hipMemcpyHostToDevice, b->stream));
+  const bool graphable = (n % 2 == 0) && b->phase == 0;
+  if (!graphable) {
+    lbm_status s = enqueue_chain(b, n, with_res);
+    if (s) return s;
+  } else {
+    if (!b->chain || b->chain_len != n || b->chain_res != int(with_res)) {
+      drop_chain(b);
+      hipGraph_t g = nullptr;
+      HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      lbm_status s = enqueue_chain(b, n, with_res);
+      hipError_t e = hipStreamEndCapture(b->stream, &g);
+      if (s) return s;
+      HIP_TRY(e);
+      HIP_TRY(hipGraphInstantiate(&b->chain, g, nullptr, nullptr, 0));
+      HIP_TRY(hipGraphDestroy(g));
+      b->chain_len = n;
+      b->chain_res = int(with_res);
+    }
+    HIP_TRY(hipGraphLaunch(b->chain, b->stream));
+  }
+  if (n % 2) b->phase ^= 1;
+  for (lbm_ctx* c : b->members) c->phase = b->phase;
+  return LBM_OK;
+}
+
+lbm_status batch_ready(lbm_batch* b) {
+  for (lbm_ctx* c : b->members) {
+    lbm_status s = check_ready(c);
+    if (s) return s;
+    if (c->np > 0) return fail(LBM_ERR_NOT_SUPPORTED, "particles in a batch member");
+    if (c->phase != b->phase)
+      return fail(LBM_ERR_INVALID_STATE, "a member's populations were reset; reset all members together");
+  }
+  return LBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+lbm_status lbm_batch_create(lbm_batch** out, int32_t lattice, int32_t storage, uint32_t count,
+                            const uint64_t* extents, int32_t device) {
+  if (!out || !extents || count == 0) return fail(LBM_ERR_INVALID_ARG, "bad batch arguments");
+  *out = nullptr;
+  lbm_batch* b = new lbm_batch();
+  b->lattice = lattice;
+  b->storage = storage;
+  b->D = dims_of(lattice);
+  b->device = device;
+  auto bail = [&](lbm_status s) {
+    lbm_batch_destroy(b);
+    return s;
+  };
+  for (uint32_t m = 0; m < count; ++m) {
+    lbm_ctx* c = nullptr;
+    lbm_status s = lbm_ctx_create(&c, lattice, storage, extents + 3 * size_t(m), device);
+    if (s) return bail(s);
+    b->members.push_back(c);
+    b->active.push_back(1);
+    void* up = nullptr;
+    hipError_t e = alloc_zero(&up, size_t(c->D) * c->N * c->tsz(), c->stream);
+    if (e != hipSuccess) return bail(fail(LBM_ERR_OUT_OF_MEMORY, "u_prev: %s", hipGetErrorString(e)));
+    b->uprev.push_back(up);
+    (void)hipStreamSynchronize(c->stream);
+  }
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess)
+    return bail(fail(LBM_ERR_CRITICAL, "stream/event creation"));
+  hipError_t e = hipMalloc((void**)&b->d_members, count * sizeof(StepArgs));
+  if (e == hipSuccess) e = hipMalloc((void**)&b->d_tile_begin, (count + 1) * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc((void**)&b->d_uprev, count * sizeof(void*));
+  if (e == hipSuccess) e = hipMalloc((void**)&b->d_res, 2 * count * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc((void**)&b->d_t0, sizeof(uint64_t));
+  if (e == hipSuccess) e = hipHostMalloc((void**)&b->h_t0, sizeof(uint64_t), hipHostMallocDefault);
+  if (e == hipSuccess)
+    e = hipMemcpy(b->d_uprev, b->uprev.data(), count * sizeof(void*), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return bail(fail(LBM_ERR_OUT_OF_MEMORY, "batch arrays: %s", hipGetErrorString(e)));
+  *out = b;
+  return LBM_OK;
+}
+
+lbm_status lbm_batch_destroy(lbm_batch* b) {
+  if (!b) return LBM_OK;
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  drop_chain(b);
+  for (void* p : b->uprev) (void)hipFree(p);
+  for (lbm_ctx* c : b->members) lbm_ctx_destroy(c);
+  void* ptrs[] = {b->d_members, b->d_tiles, b->d_tile_begin, b->d_uprev, b->d_partial, b->d_res, b->d_t0};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (b->h_t0) (void)hipHostFree(b->h_t0);
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return LBM_OK;
+}
+
+lbm_ctx* lbm_batch_member(lbm_batch* b, uint32_t i) {
+  if (!b || i >= b->members.size()) return nullptr;
+  // set-up through the member may change pointers (lazily allocated fields)
+  b->tiles_dirty = true;
+  return b->members[i];
+}
+
+lbm_status lbm_batch_step(lbm_batch* b, uint64_t t0, uint64_t nsteps) {
+  if (!b) return fail(LBM_ERR_INVALID_ARG, "null batch");
+  std::fill(b->active.begin(), b->active.end(), 1);
+  b->tiles_dirty = true;
+  lbm_status s = batch_ready(b);
+  if (s) return s;
+  HIP_TRY(hipSetDevice(b->device));
+  if (b->tiles_dirty) {
+    s = build_tiles(b);
+    if (s) return s;
+  }
+  HIP_TRY(hipEventRecord(b->ev0, b->stream));
+  s = run_chain(b, t0, nsteps, false);
+  if (s) return s;
+  HIP_TRY(hipEventRecord(b->ev1, b->stream));
+  HIP_TRY(hipEventSynchronize(b->ev1));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+  b->last_ms = ms;
+  uint64_t cells = 0;
+  for (lbm_ctx* c : b->members) cells += c->N;
+  b->last_updates = cells * nsteps;
+  return LBM_OK;
+}
+
+lbm_status lbm_batch_run(lbm_batch* b, uint64_t t0, double tol, uint64_t check_every,
+                         uint64_t max_steps, uint64_t* steps_out, double* res_out) {
+  if (!b) return fail(LBM_ERR_INVALID_ARG, "null batch");
+  if (check_every < 2 || check_every % 2) return fail(LBM_ERR_INVALID_ARG, "check_every must be even and >= 2");
+  if (b->phase != 0) return fail(LBM_ERR_INVALID_STATE, "batch runs start from the natural layout");
+  std::fill(b->active.begin(), b->active.end(), 1);
+  b->tiles_dirty = true;
+  lbm_status s = batch_ready(b);
+  if (s) return s;
+  HIP_TRY(hipSetDevice(b->device));
+  const uint32_t M = uint32_t(b->members.size());
+  std::vector<uint64_t> steps(M, 0);
+  std::vector<double> res(M, 0.0), sums(2 * M);
+  uint64_t done = 0, updates = 0;
+  float total_ms = 0.f;
+  while (done < max_steps) {
+    const uint64_t n = std::min<uint64_t>(check_every, max_steps - done) & ~uint64_t(1);
+    if (n < 2) break;
+    uint64_t cells = 0;
+    for (uint32_t m = 0; m < M; ++m)
+      if (b->active[m]) cells += b->members[m]->N;
+    HIP_TRY(hipEventRecord(b->ev0, b->stream));
+    s = run_chain(b, t0 + done, n, true);
+    if (s) return s;
+    HIP_TRY(hipEventRecord(b->ev1, b->stream));
+    HIP_TRY(hipMemcpyAsync(sums.data(), b->d_res, 2 * M * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    total_ms += ms;
+    updates += cells * n;
+    done += n;
+    bool changed = false, any = false;
+    for (uint32_t m = 0; m < M; ++m) {
+      if (!b->active[m]) continue;
+      steps[m] += n;
+      const double r = sums[2 * m + 1] > 0 ? std::sqrt(sums[2 * m] / sums[2 * m + 1]) : 0.0;
+      res[m] = r;
+      if (!(r >= tol)) {  // converged (or NaN: stop and report it)
+        b->active[m] = 0;
+        changed = true;
+      } else {
+        any = true;
+      }
+    }
+    if (!any) break;
+    if (changed) b->tiles_dirty = true;
+  }
+  b->last_ms = total_ms;
+  b->last_updates = updates;
+  if (steps_out) std::copy(steps.begin(), steps.end(), steps_out);
+  if (res_out) std::copy(res.begin(), res.end(), res_out);
+  std::fill(b->active.begin(), b->active.end(), 1);
+  b->tiles_dirty = true;
+  return LBM_OK;
+}
+
+lbm_status lbm_batch_set_tuning(lbm_batch* b, const lbm_batch_tuning* t) {
+  if (!b || !t) return fail(LBM_ERR_INVALID_ARG, "null argument");
+  if (t->block < 64 || t->block > 256 || (t->block & 63) || t->rows < 1 || t->rows > 16 || t->grid < 0)
+    return fail(LBM_ERR_INVALID_ARG, "tuning out of range");
+  b->shape.block = t->block;
+  b->shape.rows = t->rows;
+  b->shape.grid = t->grid;
+  b->tiles_dirty = true;
+  return LBM_OK;
+}
+
+lbm_status lbm_batch_autotune(lbm_batch* b, lbm_batch_tuning* chosen) {
+  if (!b) return fail(LBM_ERR_INVALID_ARG, "null batch");
+  lbm_status s = batch_ready(b);
+  if (s) return s;
+  if (b->phase != 0) return fail(LBM_ERR_INVALID_STATE, "autotune from the natural layout");
+  HIP_TRY(hipSetDevice(b->device));
+  // keep the populations and macros: trials advance the state
+  std::vector<void*> saved;
+  auto restore = [&]() -> lbm_status {
+    size_t k = 0;
+    for (lbm_ctx* c : b->members) {
+      const size_t fb = size_t(c->Q) * c->Np * storage_bytes(c->storage);
+      HIP_TRY(hipMemcpyAsync(c->f, saved[k++], fb, hipMemcpyDeviceToDevice, b->stream));
+      HIP_TRY(hipMemcpyAsync(c->u, saved[k++], size_t(c->D) * c->N * c->tsz(), hipMemcpyDeviceToDevice, b->stream));
+      HIP_TRY(hipMemcpyAsync(c->rho, saved[k++], c->N * c->tsz(), hipMemcpyDeviceToDevice, b->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    b->phase = 0;
+    for (lbm_ctx* c : b->members) c->phase = 0;
+    return LBM_OK;
+  };
+  for (lbm_ctx* c : b->members) {
+    const size_t sizes[3] = {size_t(c->Q) * c->Np * storage_bytes(c->storage), size_t(c->D) * c->N * c->tsz(),
+                             c->N * c->tsz()};
+    void* srcs[3] = {c->f, c->u, c->rho};
+    for (int k = 0; k < 3; ++k) {
+      void* p = nullptr;
+      HIP_TRY(hipMalloc(&p, sizes[k]));
+      HIP_TRY(hipMemcpyAsync(p, srcs[k], sizes[k], hipMemcpyDeviceToDevice, b->stream));
+      saved.push_back(p);
+    }
+  }
+  int cus = 0;
+  HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device));
+  const BatchShape keep = b->shape;
+  BatchShape best = keep;
+  double best_ms = 1e30;
+  const int blocks[3] = {64, 128, 256}, rows_c[3] = {1, 2, 4}, grids[4] = {0, 4, 8, 16};
+  for (int bi = 0; bi < 3; ++bi)
+    for (int ri = 0; ri < 3; ++ri)
+      for (int gi = 0; gi < 4; ++gi) {
+        b->shape.block = blocks[bi];
+        b->shape.rows = rows_c[ri];
+        b->shape.grid = grids[gi] == 0 ? 0 : grids[gi] * cus;
+        b->tiles_dirty = true;
+        s = build_tiles(b);
+        if (s) return s;
+        s = run_chain(b, 0, 2, false);  // warm-up
+        if (s) return s;
+        HIP_TRY(hipEventRecord(b->ev0, b->stream));
+        s = run_chain(b, 2, 8, false);
+        if (s) return s;
+        HIP_TRY(hipEventRecord(b->ev1, b->stream));
+        HIP_TRY(hipEventSynchronize(b->ev1));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+        if (ms / 8.0 < best_ms) {
+          best_ms = ms / 8.0;
+          best = b->shape;
+        }
+      }
+  s = restore();
+  if (s) return s;
+  for (void* p : saved) (void)hipFree(p);
+  b->shape = best;
+  b->tiles_dirty = true;
+  if (chosen) {
+    chosen->block = best.block;
+    chosen->rows = best.rows;
+    chosen->grid = best.grid;
+    chosen->ms_per_step = best_ms;
+  }
+  return LBM_OK;
+}
+
+lbm_status lbm_batch_stats(lbm_batch* b, double* ms, uint64_t* updates) {
+  if (!b) return fail(LBM_ERR_INVALID_ARG, "null batch");
+  if (ms) *ms = b->last_ms;
+  if (updates) *updates = b->last_updates;
+  return LBM_OK;
+}
+
+}  // extern "C"

@@ -266,13 +266,12 @@ __global__ __launch_bounds__(256) void k_aa_step(StepArgs a) {
 // one row (blockIdx.y = y, blockIdx.z = z), so the row offsets of the ±1
 // neighbour rows and their validity are wavefront-uniform, and the partial
 // cache lines that the ±1 shifts along x create stay inside one block.
+// One cell of one row: y and z are uniform across the calling wavefront,
+// x varies per lane (x < nx for every lane that calls).
 template <class L, class S, class T, bool WB, bool ODD>
-__global__ __launch_bounds__(1024) void k_aa_row(StepArgs a) {
+__device__ __forceinline__ void aa_row_cell(const StepArgs& a, int x, int y, int z) {
   constexpr int Q = L::Q;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  const int y = blockIdx.y, z = blockIdx.z;
-  if (x >= nx) return;
   const uint32_t sxy = uint32_t(nx) * uint32_t(ny);
   const bool px = a.periodic & 1u, py = a.periodic & 2u, pz = a.periodic & 4u;
   // wavefront-uniform: the base of every (plane, neighbour row) pair, and
@@ -332,6 +331,12 @@ __global__ __launch_bounds__(1024) void k_aa_row(StepArgs a) {
       if (!(VX[jx] && VY[jy] && VZ[jz])) Store<S, T>::st(rowp(i, 1, 1) + X[1], T(0), T(L::w[i]));
     }
   }
+}
+
+template <class L, class S, class T, bool WB, bool ODD>
+__global__ __launch_bounds__(1024) void k_aa_row(StepArgs a) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x < a.nx) aa_row_cell<L, S, T, WB, ODD>(a, x, blockIdx.y, blockIdx.z);
 }
 
 // ---- vectorised variant ------------------------------------------------------
@@ -562,6 +567,137 @@ static hipError_t launch_vec(bool odd, const StepArgs& a, hipStream_t s) {
     hipLaunchKernelGGL((k_aa_vec<L, T, WB, true, V>), grid, block, 0, s, a);
   else
     hipLaunchKernelGGL((k_aa_vec<L, T, WB, false, V>), grid, block, 0, s, a);
+  return hipGetLastError();
+}
+
+// ---- batched ensembles --------------------------------------------------------
+// Many independent lattices of uneven size, one launch per step: persistent
+// blocks stride over a work list of row tiles ordered member by member; each
+// tile applies exactly the single-lattice row body to its member's fields.
+// The step index is read from device memory (t0 + k) so a chain of steps can
+// be captured once in a hipGraph and replayed.
+template <class L, class S, class T, bool WB, bool ODD>
+__global__ __launch_bounds__(256) void k_aa_batch(BatchArgs b) {
+  const uint64_t t = *b.t0 + b.k;
+  for (uint32_t tile = blockIdx.x; tile < b.ntiles; tile += gridDim.x) {
+    const Tile tl = b.tiles[tile];
+    StepArgs a = b.members[tl.member];
+    a.t = t;
+    a.write_macros = b.write_macros != 0;
+    if (b.write_macros == 2) a.u = b.u_prev[tl.member];
+    const int x = int(tl.x0 + threadIdx.x);
+    for (uint32_t r = 0; r < b.rows; ++r) {
+      const int y = int(tl.y + r);
+      if (y >= a.ny) break;
+      if (x < a.nx) aa_row_cell<L, S, T, WB, ODD>(a, x, y, int(tl.z));
+    }
+  }
+}
+
+template <class T, int D>
+__global__ __launch_bounds__(256) void k_batch_residual(BatchArgs b, double* partial) {
+  __shared__ double red[4][2];
+  for (uint32_t tile = blockIdx.x; tile < b.ntiles; tile += gridDim.x) {
+    const Tile tl = b.tiles[tile];
+    const StepArgs& a = b.members[tl.member];
+    const T* u = static_cast<const T*>(a.u);
+    const T* up = static_cast<const T*>(b.u_prev[tl.member]);
+    double dsum = 0.0, usum = 0.0;
+    const int x = int(tl.x0 + threadIdx.x);
+    for (uint32_t r = 0; r < b.rows; ++r) {
+      const int y = int(tl.y + r);
+      if (y >= a.ny) break;
+      if (x < a.nx) {
+        const uint32_t c = uint32_t(x) + uint32_t(a.nx) * (uint32_t(y) + uint32_t(a.ny) * tl.z);
+        for (int d = 0; d < D; ++d) {
+          const double v = double(u[size_t(d) * a.N + c]);
+          const double w = double(up[size_t(d) * a.N + c]);
+          dsum += (v - w) * (v - w);
+          usum += v * v;
+        }
+      }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      dsum += __shfl_xor(dsum, o, 64);
+      usum += __shfl_xor(usum, o, 64);
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) {
+      red[wv][0] = dsum;
+      red[wv][1] = usum;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+      double s = 0.0;
+      for (int w2 = 0; w2 < int((blockDim.x + 63) >> 6); ++w2) s += red[w2][threadIdx.x];
+      partial[size_t(tile) * 2 + threadIdx.x] = s;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_batch_residual_sum(const double* partial, const uint32_t* tile_begin,
+                                     uint32_t members, double* out) {
+  const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= members) return;
+  double d = 0.0, u = 0.0;
+  for (uint32_t k = tile_begin[m]; k < tile_begin[m + 1]; ++k) {
+    d += partial[size_t(k) * 2];
+    u += partial[size_t(k) * 2 + 1];
+  }
+  out[2 * m] = d;
+  out[2 * m + 1] = u;
+}
+
+template <class L, class S, class T, bool WB>
+static hipError_t launch_batch_t(bool odd, const BatchArgs& b, const BatchShape& sh, hipStream_t s) {
+  const uint32_t grid = sh.grid > 0 ? uint32_t(sh.grid) : b.ntiles;
+  if (b.ntiles == 0) return hipSuccess;
+  if (odd)
+    hipLaunchKernelGGL((k_aa_batch<L, S, T, WB, true>), dim3(grid), dim3(sh.block), 0, s, b);
+  else
+    hipLaunchKernelGGL((k_aa_batch<L, S, T, WB, false>), dim3(grid), dim3(sh.block), 0, s, b);
+  return hipGetLastError();
+}
+
+template <class L>
+static hipError_t launch_batch_l(int storage, bool odd, const BatchArgs& b, const BatchShape& sh,
+                                 hipStream_t s) {
+  switch (storage) {
+    case LBM_F64: return launch_batch_t<L, double, double, true>(odd, b, sh, s);
+    case LBM_F32: return launch_batch_t<L, float, float, true>(odd, b, sh, s);
+    case LBM_F16S: return launch_batch_t<L, __half, float, false>(odd, b, sh, s);
+    case LBM_BF16S: return launch_batch_t<L, __hip_bfloat16, float, false>(odd, b, sh, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_aa_batch(int lattice, int storage, bool odd, const BatchArgs& b,
+                           const BatchShape& sh, hipStream_t s) {
+  switch (lattice) {
+    case LBM_D2Q9: return launch_batch_l<D2Q9>(storage, odd, b, sh, s);
+    case LBM_D3Q19: return launch_batch_l<D3Q19>(storage, odd, b, sh, s);
+    case LBM_D3Q27: return launch_batch_l<D3Q27>(storage, odd, b, sh, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_batch_residual(int dims, int storage, const BatchArgs& b, const BatchShape& sh,
+                                 double* partial, const uint32_t* tile_begin, uint32_t members,
+                                 double* out, hipStream_t s) {
+  if (b.ntiles == 0) return hipSuccess;
+  const uint32_t grid = sh.grid > 0 ? uint32_t(sh.grid) : b.ntiles;
+  const bool dbl = storage == LBM_F64;
+  if (dims == 2) {
+    if (dbl) hipLaunchKernelGGL((k_batch_residual<double, 2>), dim3(grid), dim3(sh.block), 0, s, b, partial);
+    else hipLaunchKernelGGL((k_batch_residual<float, 2>), dim3(grid), dim3(sh.block), 0, s, b, partial);
+  } else {
+    if (dbl) hipLaunchKernelGGL((k_batch_residual<double, 3>), dim3(grid), dim3(sh.block), 0, s, b, partial);
+    else hipLaunchKernelGGL((k_batch_residual<float, 3>), dim3(grid), dim3(sh.block), 0, s, b, partial);
+  }
+  hipLaunchKernelGGL(k_batch_residual_sum, dim3((members + 255) / 256), dim3(256), 0, s, partial,
+                     tile_begin, members, out);
   return hipGetLastError();
 }
 

@@ -17,22 +17,27 @@ def _dp(a):
 
 
 class Lattice:
-    def __init__(self, lattice, storage, extent, device=0):
+    def __init__(self, lattice, storage, extent, device=0, _handle=None):
         self.L = load_library()
         self.lattice, self.storage = lattice, storage
         self.D, self.Q = abi.LATTICE_DQ[lattice]
         ext = list(extent) + [1] * (3 - len(extent))
         self.extent = tuple(ext[: self.D])
         self.N = int(np.prod(self.extent))
-        h = C.c_void_p()
-        check(self.L.lbm_ctx_create(C.byref(h), lattice, storage, (C.c_uint64 * 3)(*ext), device))
-        self._h = h
+        self._owned = _handle is None
+        if _handle is None:
+            h = C.c_void_p()
+            check(self.L.lbm_ctx_create(C.byref(h), lattice, storage, (C.c_uint64 * 3)(*ext), device))
+            self._h = h
+        else:
+            self._h = _handle
         self.t = 0
         self.n_particles = 0
 
     def close(self):
         if getattr(self, "_h", None):
-            self.L.lbm_ctx_destroy(self._h)
+            if self._owned:
+                self.L.lbm_ctx_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -161,3 +166,68 @@ def spheroid_mass(dims, storage, radius, density):
     m, i = C.c_double(), C.c_double()
     check(L.lbm_spheroid_mass(dims, storage, radius, density, C.byref(m), C.byref(i)))
     return m.value, i.value
+
+
+class Batch:
+    """Many independent lattices of uneven size stepped by one launch per
+    step (lbm_batch_*).  ``members[i]`` is a Lattice view of member i for
+    set-up and read-back."""
+
+    def __init__(self, lattice, storage, extents, device=0):
+        self.L = load_library()
+        self.lattice, self.storage = lattice, storage
+        ext = []
+        for e in extents:
+            e = list(e) + [1] * (3 - len(e))
+            ext.extend(e[:3])
+        arr = (C.c_uint64 * len(ext))(*ext)
+        h = C.c_void_p()
+        check(self.L.lbm_batch_create(C.byref(h), lattice, storage, len(extents), arr, device))
+        self._h = h
+        self.members = []
+        for i, e in enumerate(extents):
+            mh = self.L.lbm_batch_member(self._h, i)
+            self.members.append(Lattice(lattice, storage, e, _handle=C.c_void_p(mh)))
+        self.t = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            for m in self.members:
+                m._h = None
+            self.L.lbm_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, n, t0=None):
+        t = self.t if t0 is None else t0
+        check(self.L.lbm_batch_step(self._h, t, n))
+        self.t = t + n
+
+    def run(self, tol, check_every=100, max_steps=100000, t0=None):
+        t = self.t if t0 is None else t0
+        M = len(self.members)
+        steps = (C.c_uint64 * M)()
+        res = (C.c_double * M)()
+        check(self.L.lbm_batch_run(self._h, t, tol, check_every, max_steps, steps, res))
+        return np.array(steps[:], dtype=np.uint64), np.array(res[:])
+
+    def autotune(self):
+        t = abi.BatchTuning()
+        check(self.L.lbm_batch_autotune(self._h, C.byref(t)))
+        return {"block": t.block, "rows": t.rows, "grid": t.grid, "ms_per_step": t.ms_per_step}
+
+    def set_tuning(self, block, rows, grid):
+        t = abi.BatchTuning()
+        t.block, t.rows, t.grid = block, rows, grid
+        check(self.L.lbm_batch_set_tuning(self._h, C.byref(t)))
+
+    def stats(self):
+        ms = C.c_double()
+        up = C.c_uint64()
+        check(self.L.lbm_batch_stats(self._h, C.byref(ms), C.byref(up)))
+        return ms.value, up.value

@@ -104,3 +104,34 @@ def restitution_to_zeta(e):
     ζ = −ln e / sqrt(π² + ln² e)."""
     le = np.log(e)
     return -le / np.sqrt(np.pi ** 2 + le ** 2)
+
+
+def c4_member(rng, side_lo=64, side_hi=512, frac=(0.05, 0.20), r_range=(4.0, 24.0),
+              u_range=(0.01, 0.08), tau_range=(0.55, 1.2)):
+    """One ensemble member of the batched config: D2Q9 channel of uneven
+    sides, seeded random circular bounce-back obstacles covering a target
+    solid fraction, parabolic Zou/He velocity inlet (u_max drawn), Zou/He
+    pressure outlet at rho = 1, BGK with tau drawn.  Returns
+    (extent, flags, ops, inlet_velocity, params)."""
+    nx = int(rng.integers(side_lo, side_hi + 1))
+    ny = int(rng.integers(side_lo, side_hi + 1))
+    u_max = float(rng.uniform(*u_range))
+    tau = float(rng.uniform(*tau_range))
+    target = float(rng.uniform(*frac))
+    flags = channel_flags_2d(nx, ny).reshape(ny, nx)
+    yy, xx = np.mgrid[0:ny, 0:nx]
+    solid = np.zeros((ny, nx), bool)
+    interior = (xx >= 3) & (xx <= nx - 4) & (yy >= 1) & (yy <= ny - 2)
+    tries = 0
+    while solid[interior].mean() < target and tries < 500:
+        tries += 1
+        r = float(rng.uniform(*r_range))
+        r = min(r, 0.3 * min(nx, ny))
+        cx = float(rng.uniform(3 + r, nx - 4 - r)) if nx - 7 > 2 * r else nx / 2
+        cy = float(rng.uniform(1 + r, ny - 2 - r)) if ny - 3 > 2 * r else ny / 2
+        solid |= ((xx - cx) ** 2 + (yy - cy) ** 2 < r * r) & interior
+    flags[solid] = 1
+    ops = c1_ops(tau, u_max)
+    bcu = parabolic_inlet(nx, ny, u_max)
+    return (nx, ny), flags.reshape(-1), ops, bcu, {"u_max": u_max, "tau": tau,
+                                                    "solid": float(solid[interior].mean())}

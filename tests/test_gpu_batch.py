@@ -1,0 +1,87 @@
+"""Batched ensembles (lbm_batch_*): every member of a ragged batch stepped by
+one launch per step equals its own oracle run bit for bit; the GPU's FP64
+residuals equal a recomputation from two consecutive oracle states; the
+autotuner leaves the state untouched."""
+import numpy as np
+import pytest
+
+from lbm_mirror_amd import abi
+from helpers import c4_member
+
+pytestmark = pytest.mark.gpu
+
+
+def _members(seed, n, side_lo=20, side_hi=90):
+    rng = np.random.default_rng(seed)
+    return [c4_member(rng, side_lo, side_hi, r_range=(2.0, 6.0), tau_range=(0.6, 1.0),
+                      u_range=(0.01, 0.05)) for _ in range(n)]
+
+
+def _setup(sim, m):
+    ext, flags, ops, bcu, _ = m
+    sim.set_flags(flags)
+    for fl, op in ops.items():
+        sim.set_operator(fl, op)
+    sim.set_boundary_velocity(bcu)
+    sim.init_equilibrium(1.0, (0.0, 0.0))
+
+
+@pytest.mark.parametrize("st", [abi.F64, abi.F32])
+def test_batch_members_bitwise(hip, orc, st):
+    from lbm_mirror_amd.lattice import Batch
+    ms = _members(2024, 7)
+    b = Batch(abi.D2Q9, st, [m[0] for m in ms])
+    for mem, m in zip(b.members, ms):
+        _setup(mem, m)
+    b.step(37)   # odd: ends in the swapped layout, no graph
+    b.step(64)   # even from the swapped layout
+    b.step(64)   # even from the natural layout: graph-captured chain
+    for mem, m in zip(b.members, ms):
+        o = orc.Sim(abi.D2Q9, st, m[0])
+        _setup(o, m)
+        o.step(165)
+        assert np.array_equal(mem.get_populations(), o.get_populations())
+        rg, ug = mem.get_macros()
+        ro, uo = o.get_macros()
+        assert np.array_equal(rg, ro) and np.array_equal(ug, uo)
+
+
+def test_batch_residual_matches_recomputation(hip, orc):
+    from lbm_mirror_amd.lattice import Batch
+    ms = _members(7, 4)
+    b = Batch(abi.D2Q9, abi.F64, [m[0] for m in ms])
+    for mem, m in zip(b.members, ms):
+        _setup(mem, m)
+    steps, res = b.run(tol=1e-3, check_every=40, max_steps=400)
+    assert np.all(steps > 0) and np.all(steps % 40 == 0)
+    for i, m in enumerate(ms):
+        o = orc.Sim(abi.D2Q9, abi.F64, m[0])
+        _setup(o, m)
+        o.step(int(steps[i]) - 1)
+        _, u1 = o.get_macros()
+        o.step(1)
+        _, u2 = o.get_macros()
+        want = np.sqrt(np.sum((u2 - u1) ** 2) / np.sum(u2 ** 2))
+        assert abs(res[i] - want) <= 1e-12 * want
+        # converged members stopped at their check, others ran to the limit
+        assert (res[i] < 1e-3) or steps[i] == 400
+        # the member's state is that of its last step
+        assert np.array_equal(b.members[i].get_macros()[1], u2)
+
+
+def test_batch_autotune_restores_state(hip):
+    from lbm_mirror_amd.lattice import Batch
+    ms = _members(11, 5, side_lo=40, side_hi=200)
+    b = Batch(abi.D2Q9, abi.F32, [m[0] for m in ms])
+    for mem, m in zip(b.members, ms):
+        _setup(mem, m)
+    b.step(10)
+    before = [mem.get_populations() for mem in b.members]
+    cfg = b.autotune()
+    assert cfg["block"] in (64, 128, 256) and cfg["rows"] in (1, 2, 4) and cfg["ms_per_step"] > 0
+    after = [mem.get_populations() for mem in b.members]
+    for x, y in zip(before, after):
+        assert np.array_equal(x, y)
+    b.step(10)
+    ms_, upd = b.stats()
+    assert upd == 10 * sum(int(np.prod(m[0])) for m in ms) and ms_ > 0
