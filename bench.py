@@ -66,6 +66,237 @@ def cpu_baseline(ext, steps=2, threads=None):
                       f"oracle (reference arithmetic, OpenMP) on {threads} threads"}
 
 
+
+def bench_c1(args):
+    """Config 1: D2Q9 FP64 channel 256x64, parabolic Zou/He inlet u_max 0.05,
+    pressure outlet, tau 0.8, from rest; reports MLUPS and the developed
+    centre-line velocity against the analytic parabola (flux-normalised)."""
+    import numpy as np
+    from lbm_mirror_amd import abi
+    from lbm_mirror_amd.lattice import Lattice
+    from helpers import channel_flags_2d, c1_ops, parabolic_inlet
+    nx, ny = 256, 64
+    lat = Lattice(abi.D2Q9, abi.F64, (nx, ny))
+    lat.set_flags(channel_flags_2d(nx, ny))
+    for fl, op in c1_ops(0.8, 0.05).items():
+        lat.set_operator(fl, op)
+    lat.set_boundary_velocity(parabolic_inlet(nx, ny, 0.05))
+    lat.init_equilibrium(1.0, (0.0, 0.0))
+    steps = args.steps if args.steps != 1000 else 20000
+    lat.step(steps, t0=0)
+    lat.sync()
+    ms = lat.last_step_ms()
+    rho, u = lat.get_macros()
+    ux = u[0].reshape(ny, nx)[:, 3 * nx // 4]
+    y = np.arange(ny)
+    prof = 4 * (y - 0.5) * (ny - 1.5 - y) / (ny - 2.0) ** 2
+    inner = slice(1, ny - 1)
+    scale = ux[inner].sum() / prof[inner].sum()
+    err = float(np.max(np.abs(ux[inner] - scale * prof[inner])) / scale)
+    return {"metric": "MLUPS (D2Q9 FP64 Poiseuille channel)", "value": round(nx * ny * steps / (ms * 1e3), 1),
+            "unit": "MLUPS", "n_gpus": 1, "steps": steps, "warmup": 0,
+            "ms_per_step": round(ms / steps, 6), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "D2Q9 BGK channel 256x64, Zou/He velocity inlet u_max 0.05, "
+                                   "pressure outlet rho 1, BB walls, tau 0.8, from rest"},
+            "roofline": {"bound": "hbm", "achieved": round(nx * ny * 144 / (ms / steps * 1e-3) / 1e9, 1),
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(nx * ny * 144 / (ms / steps * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                         "traffic": None},
+            "analytic": {"fitted_u_max": round(scale, 6),
+                         "max_shape_error_rel": round(err, 5)}}
+
+
+def bench_c3(args, storage):
+    """Config 3: D2Q9 PSM particle-laden channel 2048x512, 200 discs r~U[8,16]
+    placed by seeded rejection, density 1.5, gravity 1e-6, DEM k=1e3,
+    restitution 0.9 (zeta from e), tau 0.7; full particle pipeline each step."""
+    import numpy as np
+    from lbm_mirror_amd import abi
+    from lbm_mirror_amd.lattice import Lattice, spheroid_mass
+    from helpers import channel_flags_2d, seeded_discs, restitution_to_zeta
+    nx, ny = 2048, 512
+    lat = Lattice(abi.D2Q9, storage, (nx, ny))
+    lat.set_flags(channel_flags_2d(nx, ny))
+    ops = {1: abi.op_desc(boundary=abi.BC_BOUNCE_BACK),
+           2: abi.op_desc(collision=abi.COLL_PSM, tau=0.7),
+           3: abi.op_desc(boundary=abi.BC_ZOUHE_PRESSURE_X, collision=abi.COLL_BGK, east=1, tau=0.7,
+                          ramp=(1.0, 0.001 / 64, 1.001, 64)),
+           4: abi.op_desc(boundary=abi.BC_ZOUHE_PRESSURE_X, collision=abi.COLL_BGK, east=0,
+                          tau=0.7, rho=1.0)}
+    for fl, op in ops.items():
+        lat.set_operator(fl, op)
+    lat.init_equilibrium(1.0, (0.0, 0.0))
+    discs = seeded_discs(200, (nx, ny), 8.0, 16.0, seed=20260103)
+    parts = []
+    for (x, y, r) in discs:
+        p = abi.Particle()
+        p.position[0] = p.position_old[0] = x
+        p.position[1] = p.position_old[1] = y
+        p.radius, p.density = r, 1.5
+        p.mass, p.inertia = spheroid_mass(2, storage if storage in (abi.F64, abi.F32) else abi.F32, r, 1.5)
+        parts.append(p)
+    prm = abi.particle_params(k_n=1e3, zeta_n=restitution_to_zeta(0.9), walls=True, sub_steps=16,
+                              body_acceleration=(0.0, -1e-6), eps=1.5, max_radius=16.0)
+    lat.particles_set(parts, prm)
+    steps = args.steps if args.steps != 1000 else 2000
+    lat.step(args.warmup, t0=0)
+    lat.sync()
+    lat.step(steps, t0=args.warmup)
+    lat.sync()
+    ms = lat.last_step_ms()
+    ps = lat.particles_get()
+    finite = all(np.isfinite(q.position[0]) and np.isfinite(q.position[1]) for q in ps)
+    sb = {abi.F32: 4, abi.F16S: 2, abi.BF16S: 2}[storage]
+    return {"metric": "MLUPS (D2Q9 PSM particle-laden channel, full step incl. particles)",
+            "value": round(nx * ny * steps / (ms * 1e3), 1), "unit": "MLUPS", "n_gpus": 1,
+            "steps": steps, "warmup": args.warmup, "ms_per_step": round(ms / steps, 5),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": {abi.F32: "f32", abi.F16S: "f16-storage/f32", abi.BF16S: "bf16-storage/f32"}[storage],
+            "data": "synthetic",
+            "config": {"workload": "D2Q9 PSM 2048x512, 200 discs r~U[8,16], rho_p 1.5, g 1e-6, "
+                                   "DEM k 1e3 e 0.9, tau 0.7", "particles_finite": bool(finite)},
+            "roofline": {"bound": "hbm",
+                         "achieved": round(nx * ny * 2 * 9 * sb / (ms / steps * 1e-3) / 1e9, 1),
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(nx * ny * 2 * 9 * sb / (ms / steps * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                         "traffic": None}}
+
+
+def bench_c4(args):
+    """Config 4: 256 D2Q9 lattices, sides ~U[64,512], seeded obstacles 5-20%
+    solid (r~U[4,24]), u_max~U[0.01,0.08], tau~U[0.55,1.2]; one launch per
+    step over all members (autotuned tiles), run towards residual 1e-6."""
+    import numpy as np
+    from lbm_mirror_amd import abi
+    from lbm_mirror_amd.lattice import Batch
+    from helpers import c4_member
+    rng = np.random.default_rng(20260104)
+    ms_ = [c4_member(rng) for _ in range(args.members)]
+    b = Batch(abi.D2Q9, abi.F32, [m[0] for m in ms_])
+    for mem, m in zip(b.members, ms_):
+        ext, flags, ops, bcu, _ = m
+        mem.set_flags(flags)
+        for fl, op in ops.items():
+            mem.set_operator(fl, op)
+        mem.set_boundary_velocity(bcu)
+        mem.init_equilibrium(1.0, (0.0, 0.0))
+    tune = b.autotune()
+    max_steps = args.steps if args.steps != 1000 else 20000
+    steps, res = b.run(tol=1e-6, check_every=200, max_steps=max_steps)
+    ms, upd = b.stats()
+    cells = sum(int(np.prod(m[0])) for m in ms_)
+    return {"metric": "aggregate MLUPS (256 ragged D2Q9 lattices, one launch per step)",
+            "value": round(upd / (ms * 1e3), 1), "unit": "MLUPS", "n_gpus": 1,
+            "steps": int(steps.max()), "warmup": 0, "ms_per_step": None,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic",
+            "config": {"workload": f"{args.members} D2Q9 lattices, sides U[64,512], {cells} cells",
+                       "tuning": tune, "converged_1e-6": int(np.sum(res < 1e-6)),
+                       "median_steps": float(np.median(steps)), "wall_device_ms": round(ms, 2)},
+            "roofline": {"bound": "hbm", "achieved": round(upd * 72 / (ms * 1e-3) / 1e9, 1),
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(upd * 72 / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None}}
+
+
+def _c5_batch(members, warm=None):
+    from lbm_mirror_amd import abi
+    from lbm_mirror_amd.lattice import Batch
+    from lbm_mirror_amd.warmstart import init_equilibrium_field
+    b = Batch(abi.D2Q9, abi.F32, [m[0] for m in members])
+    for i, (mem, m) in enumerate(zip(b.members, members)):
+        ext, flags, ops, bcu, _ = m
+        mem.set_flags(flags)
+        for fl, op in ops.items():
+            mem.set_operator(fl, op)
+        mem.set_boundary_velocity(bcu)
+        if warm is None:
+            mem.init_equilibrium(1.0, (0.0, 0.0))
+        else:
+            init_equilibrium_field(mem, warm[0][i], warm[1][i])
+    return b
+
+
+def bench_c5(args):
+    """Config 5: learned warm start on 256x256 lattices of the ensemble
+    distribution.  The batched LBM generates the training steady states
+    (--train-samples, to residual --train-tol), the network trains on the GPU
+    (--train-steps Adam steps), and held-out members (--test-samples) run to
+    residual 1e-6 from rest (cold) and from the equilibrium of the prediction
+    (warm).  value = cold / warm wall time to steady state (inference
+    included in warm)."""
+    import numpy as np
+    from helpers import c4_member
+    from lbm_mirror_amd.warmstart import Net
+    side, tol = 256, 1e-6
+    t_all = time.perf_counter()
+    rng = np.random.default_rng(20260106)
+    n_train, n_test = args.train_samples, args.test_samples
+    net = Net(side, side, (32, 64, 96, 160), batch=16, capacity=n_train, seed=7)
+    gen_ms, gen_updates = 0.0, 0
+    chunk = 256
+    while net.sample_count() < n_train:
+        kept = 0
+        mem = [c4_member(rng, side_lo=side, side_hi=side) for _ in range(chunk)]
+        b = _c5_batch(mem)
+        steps, res = b.run(tol=args.train_tol, check_every=1000, max_steps=args.train_max_steps)
+        ms, upd = b.stats()
+        gen_ms += ms
+        gen_updates += upd
+        for m_, m, r in zip(b.members, mem, res):
+            if r < args.train_tol and net.sample_count() < n_train:
+                rho, u = m_.get_macros()
+                net.add_sample(m[1], m[4]["u_max"], m[4]["tau"], rho, u)
+                kept += 1
+        b.close()
+        print(f"# generated {net.sample_count()} / {n_train} steady states "
+              f"({time.perf_counter() - t_all:.0f} s)", file=sys.stderr, flush=True)
+        if kept == 0:
+            break
+    k = args.train_steps
+    losses, train_ms = [], 0.0
+    for frac, lr, seed in ((0.5, 1e-3, 1), (0.3, 3e-4, 2), (0.2, 1e-4, 3)):
+        losses.append(net.train(max(1, int(k * frac)), lr=lr, seed=seed))
+        train_ms += net.last_ms()
+    losses = np.concatenate(losses)
+    print(f"# trained {args.train_steps} steps: loss {losses[:20].mean():.4f} -> "
+          f"{losses[-50:].mean():.5f} in {train_ms / 1e3:.1f} s", file=sys.stderr, flush=True)
+    test = [c4_member(rng, side_lo=side, side_hi=side) for _ in range(n_test)]
+    cold = _c5_batch(test)
+    sc, rc = cold.run(tol=tol, check_every=500, max_steps=args.test_max_steps)
+    cold_ms, _ = cold.stats()
+    t0 = time.perf_counter()
+    pred = net.predict([m[1] for m in test], [m[4]["u_max"] for m in test], [m[4]["tau"] for m in test])
+    infer_ms = (time.perf_counter() - t0) * 1e3
+    warm = _c5_batch(test, warm=pred)
+    sw, rw = warm.run(tol=tol, check_every=500, max_steps=args.test_max_steps)
+    warm_ms, _ = warm.stats()
+    ok = (rc < tol) & (rw < tol)
+    diffs = []
+    for i in np.flatnonzero(ok):
+        _, uc = cold.members[i].get_macros()
+        _, uw = warm.members[i].get_macros()
+        diffs.append(float(np.linalg.norm(uw - uc) / np.linalg.norm(uc)))
+    ratio_steps = float(np.median(sc[ok]) / np.median(sw[ok])) if ok.any() else None
+    ratio_time = cold_ms / (warm_ms + infer_ms)
+    return {"metric": "time-to-steady (residual 1e-6) cold / learned warm start",
+            "value": round(ratio_time, 3), "unit": "x", "n_gpus": 1, "steps": int(args.train_steps),
+            "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "bf16 (network) / f32 (LBM)", "data": "synthetic",
+            "config": {"workload": f"D2Q9 256x256 ensemble members, {net.sample_count()} LBM-generated "
+                                   f"training steady states (residual {args.train_tol:g}), {n_test} held-out",
+                       "cold_ms": round(cold_ms, 1), "warm_ms": round(warm_ms, 1),
+                       "inference_ms": round(infer_ms, 1), "train_ms": round(train_ms, 1),
+                       "generate_ms": round(gen_ms, 1), "generate_glups": round(gen_updates / max(gen_ms, 1e-9) / 1e6, 2),
+                       "median_steps_cold": float(np.median(sc[ok])) if ok.any() else None,
+                       "median_steps_warm": float(np.median(sw[ok])) if ok.any() else None,
+                       "step_ratio": ratio_steps, "converged_both": int(ok.sum()),
+                       "converged_cold": int(np.sum(rc < tol)), "converged_warm": int(np.sum(rw < tol)),
+                       "field_diff_median": float(np.median(diffs)) if diffs else None,
+                       "field_diff_max": float(np.max(diffs)) if diffs else None,
+                       "final_loss": float(losses[-50:].mean()), "wall_s": round(time.perf_counter() - t_all, 1)},
+            "roofline": None}
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,6 +306,17 @@ def main():
     ap.add_argument("--storage", default="F32", choices=["F32", "F64", "F16S", "BF16S"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--config", default="c2", choices=["c2", "c2b", "c1", "c3", "c4", "c5"],
+                    help="c2: the headline D3Q19 channel (default); c2b: D3Q27 256^3; "
+                         "c1: D2Q9 FP64 channel; c3: particle-laden; c4: ragged batch; "
+                         "c5: learned warm start")
+    ap.add_argument("--train-samples", type=int, default=1024)
+    ap.add_argument("--test-samples", type=int, default=64)
+    ap.add_argument("--train-steps", type=int, default=3000)
+    ap.add_argument("--train-tol", type=float, default=1e-4)
+    ap.add_argument("--train-max-steps", type=int, default=100000)
+    ap.add_argument("--test-max-steps", type=int, default=400000)
+    ap.add_argument("--members", type=int, default=256)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs one lattice on one GPU; multi-GPU "
@@ -87,6 +329,17 @@ def main():
     L = lm.load_library()
     if L.lbm_device_count() < 1:
         raise SystemExit("no GPU visible to liblbm_mirror_amd.so")
+    if args.config == "c2b":
+        args.lattice = "D3Q27"
+    if args.config == "c5":
+        print(json.dumps(bench_c5(args)), flush=True)
+        return
+    if args.config in ("c1", "c3", "c4"):
+        st = {"F32": abi.F32, "F16S": abi.F16S, "BF16S": abi.BF16S, "F64": abi.F32}[args.storage]
+        out = {"c1": lambda: bench_c1(args), "c3": lambda: bench_c3(args, st),
+               "c4": lambda: bench_c4(args)}[args.config]()
+        print(json.dumps(out), flush=True)
+        return
     lat_id = {"D3Q19": abi.D3Q19, "D3Q27": abi.D3Q27}[args.lattice]
     st_id = {"F32": abi.F32, "F64": abi.F64, "F16S": abi.F16S, "BF16S": abi.BF16S}[args.storage]
     ext = (512, 128, 128) if args.lattice == "D3Q19" else (256, 256, 256)

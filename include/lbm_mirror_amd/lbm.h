@@ -171,6 +171,10 @@ lbm_status lbm_get_populations(lbm_ctx* ctx, double* f);
 /* Fill every cell with eq(ρ, u) in storage precision (init of the drivers,
  * poiseulle_2d_gpu.cpp:104-134) and reset macros to (ρ, u). */
 lbm_status lbm_init_equilibrium(lbm_ctx* ctx, double rho, const double u[3]);
+/* Per-cell variant: cell x gets eq(rho[x], u[d*N + x]) and macros (rho, u).
+ * No reference counterpart; it starts a lattice from a predicted field (the
+ * learned warm start below). */
+lbm_status lbm_init_equilibrium_field(lbm_ctx* ctx, const double* rho, const double* u);
 
 /* Macros with the reference's meaning: density/velocity written by the
  * collision from the pre-collision populations of the last step; wall cells
@@ -232,7 +236,8 @@ lbm_status lbm_batch_step(lbm_batch* b, uint64_t t0, uint64_t nsteps);
 /* Run to steady state: every check_every steps (even, >= 2) each active
  * member's residual r = |u_t - u_{t-1}|_2 / |u_t|_2 over all cells is formed
  * in FP64 on the GPU; members with r < tol stop.  Returns when all stopped or
- * after max_steps.  steps_out[i]: steps member i ran; res_out[i]: its last r. */
+ * after max_steps.  steps_out[i]: steps member i ran; res_out[i]: its last r
+ * (NaN for a member whose fields stopped being finite; it stops too). */
 lbm_status lbm_batch_run(lbm_batch* b, uint64_t t0, double tol, uint64_t check_every,
                          uint64_t max_steps, uint64_t* steps_out, double* res_out);
 /* Times candidate tile shapes on the batch (state restored afterwards) and
@@ -241,6 +246,63 @@ lbm_status lbm_batch_autotune(lbm_batch* b, lbm_batch_tuning* chosen);
 lbm_status lbm_batch_set_tuning(lbm_batch* b, const lbm_batch_tuning* tuning);
 /* Device time and member cell updates of the last step/run call. */
 lbm_status lbm_batch_stats(lbm_batch* b, double* ms, uint64_t* cell_updates);
+
+/* ---- learned warm start (no reference counterpart) -------------------------
+ * A convolutional encoder-decoder on 2D lattices of height x width cells
+ * (both multiples of 8) that maps (obstacle mask, inlet u_max, tau) to the
+ * steady (ux, uy, rho).  Activations are NHWC bf16, convolutions are 3x3
+ * same-padded bf16 MFMA GEMMs accumulating in FP32; master weights and Adam
+ * state are FP32.  Layers, in parameter order, each W[3][3][Cin][Cout] then
+ * b[Cout] (w = widths):
+ *   e1a 8->w0, e1b w0->w0, pool, e2a w0->w1, e2b w1->w1, pool,
+ *   e3a w1->w2, e3b w2->w2, pool, e4a w2->w3, e4b w3->w3,
+ *   up+cat(e3b) d3 (w3+w2)->w2, up+cat(e2b) d2 (w2+w1)->w1,
+ *   up+cat(e1b) d1 (w1+w0)->w0, out w0->3 (no ReLU; every other layer ReLU).
+ * Input features per cell (lbm_net_features): fluid indicator (flag >= 2),
+ * u_max/0.08, (tau-0.5)/0.7, the inlet profile shape 4(y-1/2)(ny-3/2-y)/(ny-2)^2,
+ * x/(nx-1), y/(ny-1), log1p(P(x)) with P(x) = 12 nu (Q/u_max) sum_{x'>x} h(x')^-3
+ * / u_max (a 1D hydraulic pressure estimate, h = open cells per column, Q the
+ * inlet flux, nu = (tau-1/2)/3), and log((ny-2)/h(x)).  Targets on fluid
+ * cells: ux/u_max, uy/u_max and (rho-1)/(3 u_max^2).                         */
+typedef struct lbm_net lbm_net;
+typedef struct lbm_net_config {
+  int32_t height, width; /* lattice rows (ny) and columns (nx)              */
+  int32_t widths[4];     /* channels per level, multiples of 8              */
+  int32_t batch;         /* samples per training / inference launch         */
+  int32_t capacity;      /* samples the on-device training set may hold     */
+  uint64_t seed;         /* weight initialisation (He normal)               */
+} lbm_net_config;
+
+lbm_status lbm_net_create(lbm_net** out, const lbm_net_config* cfg, int32_t device);
+lbm_status lbm_net_destroy(lbm_net* net);
+lbm_status lbm_net_param_count(lbm_net* net, uint64_t* count);
+/* Flat FP32 parameters in the layer order above. */
+lbm_status lbm_net_get_params(lbm_net* net, float* params);
+lbm_status lbm_net_set_params(lbm_net* net, const float* params);
+/* Features of one lattice: flags (ny*nx, reference flag values) -> 8 floats
+ * per cell, cell-major (the values the network sees, before bf16 rounding). */
+lbm_status lbm_net_features(lbm_net* net, const uint8_t* flags, double u_max, double tau, float* features);
+/* Append one steady state (flags, u_max, tau, rho[N], u[2N] in the
+ * lbm_get_macros layout) to the on-device training set. */
+lbm_status lbm_net_add_sample(lbm_net* net, const uint8_t* flags, double u_max, double tau,
+                              const double* rho, const double* u);
+lbm_status lbm_net_clear_samples(lbm_net* net);
+lbm_status lbm_net_sample_count(lbm_net* net, uint32_t* count);
+/* `steps` Adam steps (beta 0.9/0.999, eps 1e-8) on minibatches drawn without
+ * replacement per epoch (seeded); losses[k] = masked MSE of step k (may be NULL). */
+lbm_status lbm_net_train(lbm_net* net, uint32_t steps, float lr, uint64_t seed, float* losses);
+/* Predict count lattices: rho[count*N], u[count*2N] (lbm_get_macros layout,
+ * solid cells rho 1, u 0). */
+lbm_status lbm_net_predict(lbm_net* net, uint32_t count, const uint8_t* flags, const double* u_max,
+                           const double* tau, double* rho, double* u);
+/* Raw forward: x[count*N*8] features -> y[count*N*3] normalised outputs. */
+lbm_status lbm_net_forward_raw(lbm_net* net, uint32_t count, const float* x, float* y);
+/* Raw loss and gradient for the current parameters: features x, targets
+ * t[count*N*3], mask m[count*N] (1 = counted); grad has param_count floats. */
+lbm_status lbm_net_loss_grad_raw(lbm_net* net, uint32_t count, const float* x, const float* t,
+                                 const float* m, double* loss, float* grad);
+/* Device time of the last train / predict call. */
+lbm_status lbm_net_last_ms(lbm_net* net, double* ms);
 
 #ifdef __cplusplus
 }

@@ -36,6 +36,10 @@ EXPORTS = [
     "lbm_dem_contact", "lbm_particle_integrate", "lbm_cell_list_overflow",
     "lbm_batch_create", "lbm_batch_destroy", "lbm_batch_member", "lbm_batch_step",
     "lbm_batch_run", "lbm_batch_autotune", "lbm_batch_set_tuning", "lbm_batch_stats",
+    "lbm_init_equilibrium_field", "lbm_net_create", "lbm_net_destroy", "lbm_net_param_count",
+    "lbm_net_get_params", "lbm_net_set_params", "lbm_net_features", "lbm_net_add_sample",
+    "lbm_net_clear_samples", "lbm_net_sample_count", "lbm_net_train", "lbm_net_predict",
+    "lbm_net_forward_raw", "lbm_net_loss_grad_raw", "lbm_net_last_ms",
 ]
 
 
@@ -90,6 +94,23 @@ def load_library(path=None):
     L.lbm_batch_autotune.argtypes = [vp, C.POINTER(abi.BatchTuning)]
     L.lbm_batch_set_tuning.argtypes = [vp, C.POINTER(abi.BatchTuning)]
     L.lbm_batch_stats.argtypes = [vp, dp, u64p]
+    L.lbm_init_equilibrium_field.argtypes = [vp, dp, dp]
+    fp = C.POINTER(C.c_float)
+    u8p = C.POINTER(C.c_uint8)
+    L.lbm_net_create.argtypes = [C.POINTER(vp), C.POINTER(abi.NetConfig), C.c_int32]
+    L.lbm_net_destroy.argtypes = [vp]
+    L.lbm_net_param_count.argtypes = [vp, u64p]
+    L.lbm_net_get_params.argtypes = [vp, fp]
+    L.lbm_net_set_params.argtypes = [vp, fp]
+    L.lbm_net_features.argtypes = [vp, u8p, C.c_double, C.c_double, fp]
+    L.lbm_net_add_sample.argtypes = [vp, u8p, C.c_double, C.c_double, dp, dp]
+    L.lbm_net_clear_samples.argtypes = [vp]
+    L.lbm_net_sample_count.argtypes = [vp, C.POINTER(C.c_uint32)]
+    L.lbm_net_train.argtypes = [vp, C.c_uint32, C.c_float, C.c_uint64, fp]
+    L.lbm_net_predict.argtypes = [vp, C.c_uint32, u8p, dp, dp, dp, dp]
+    L.lbm_net_forward_raw.argtypes = [vp, C.c_uint32, fp, fp]
+    L.lbm_net_loss_grad_raw.argtypes = [vp, C.c_uint32, fp, fp, fp, dp, fp]
+    L.lbm_net_last_ms.argtypes = [vp, dp]
     for name in EXPORTS:
         f = getattr(L, name)
         if name not in ("lbm_last_error", "lbm_version", "lbm_device_count", "lbm_batch_member"):

@@ -136,3 +136,23 @@ class BatchTuning(C.Structure):
         ("grid", C.c_int32),
         ("ms_per_step", C.c_double),
     ]
+
+
+class NetConfig(C.Structure):
+    _fields_ = [
+        ("height", C.c_int32),
+        ("width", C.c_int32),
+        ("widths", C.c_int32 * 4),
+        ("batch", C.c_int32),
+        ("capacity", C.c_int32),
+        ("seed", C.c_uint64),
+    ]
+
+
+def net_config(height, width, widths=(32, 64, 96, 160), batch=16, capacity=0, seed=1):
+    c = NetConfig()
+    c.height, c.width = height, width
+    for i in range(4):
+        c.widths[i] = widths[i]
+    c.batch, c.capacity, c.seed = batch, capacity, seed
+    return c

@@ -55,6 +55,12 @@ const int kMaxParticles = 4096;
 
 }  // namespace
 
+// Error channel shared with the other translation units of the library.
+lbm_status lbmx_report(lbm_status s, const char* msg) {
+  g_err = msg;
+  return s;
+}
+
 struct lbm_ctx {
   int lattice = 0, storage = 0, D = 2, Q = 9, device = 0;
   uint64_t n[3] = {1, 1, 1};
@@ -463,15 +469,18 @@ lbm_status lbm_get_populations(lbm_ctx* c, double* f) {
   return LBM_OK;
 }
 
-lbm_status lbm_init_equilibrium(lbm_ctx* c, double rho, const double u[3]) {
-  if (!c || !u) return fail(LBM_ERR_INVALID_ARG, "null argument");
-  // eq(ρ, u) in T with the reference's arithmetic, evaluated on the host
-  double eq[27];
-  const int st = c->storage;
+}  // extern "C"
+
+namespace {
+
+// eq(ρ, u) in T with the reference's arithmetic (equilibrium.hpp:7-48),
+// evaluated on the host; out[i] holds the value rounded to T.
+void host_equilibrium(int lattice, int storage, double rho, const double* u, double* out) {
   auto fill = [&](auto L, auto zero) {
     using LL = decltype(L);
     using T = decltype(zero);
-    T uu[3] = {T(u[0]), T(u[1]), T(u[2])};
+    T uu[3] = {T(0), T(0), T(0)};
+    for (int d = 0; d < LL::D; ++d) uu[d] = T(u[d]);
     T r = T(rho);
     T vv = T(0);
     for (int d = 0; d < LL::D; ++d) vv = T(vv + T(uu[d] * uu[d]));
@@ -481,13 +490,24 @@ lbm_status lbm_init_equilibrium(lbm_ctx* c, double rho, const double u[3]) {
       T a = T(vc * T(3.0));
       T wr = T(T(LL::w[i]) * r);
       T aa = T(a * a);
-      eq[i] = double(T(double(wr) * (((1.0 + double(a)) - 1.5 * double(vv)) + double(aa) * 0.5)));
+      out[i] = double(T(double(wr) * (((1.0 + double(a)) - 1.5 * double(vv)) + double(aa) * 0.5)));
     }
   };
-  const bool dbl = compute_double(st);
-  if (c->lattice == LBM_D2Q9) dbl ? fill(D2Q9{}, 0.0) : fill(D2Q9{}, 0.0f);
-  if (c->lattice == LBM_D3Q19) dbl ? fill(D3Q19{}, 0.0) : fill(D3Q19{}, 0.0f);
-  if (c->lattice == LBM_D3Q27) dbl ? fill(D3Q27{}, 0.0) : fill(D3Q27{}, 0.0f);
+  const bool dbl = compute_double(storage);
+  if (lattice == LBM_D2Q9) dbl ? fill(D2Q9{}, 0.0) : fill(D2Q9{}, 0.0f);
+  if (lattice == LBM_D3Q19) dbl ? fill(D3Q19{}, 0.0) : fill(D3Q19{}, 0.0f);
+  if (lattice == LBM_D3Q27) dbl ? fill(D3Q27{}, 0.0) : fill(D3Q27{}, 0.0f);
+}
+
+}  // namespace
+
+extern "C" {
+
+lbm_status lbm_init_equilibrium(lbm_ctx* c, double rho, const double u[3]) {
+  if (!c || !u) return fail(LBM_ERR_INVALID_ARG, "null argument");
+  double eq[27];
+  const int st = c->storage;
+  host_equilibrium(c->lattice, st, rho, u, eq);
   std::vector<double> h(size_t(c->Q) * c->N);
   for (int i = 0; i < c->Q; ++i) std::fill(h.begin() + size_t(i) * c->N, h.begin() + size_t(i + 1) * c->N, eq[i]);
   lbm_status s = lbm_set_populations(c, h.data());
@@ -495,6 +515,24 @@ lbm_status lbm_init_equilibrium(lbm_ctx* c, double rho, const double u[3]) {
   std::vector<double> r(c->N, to_T(st, rho)), uv(size_t(c->D) * c->N);
   for (int d = 0; d < c->D; ++d)
     std::fill(uv.begin() + size_t(d) * c->N, uv.begin() + size_t(d + 1) * c->N, to_T(st, u[d]));
+  return lbm_set_macros(c, r.data(), uv.data());
+}
+
+lbm_status lbm_init_equilibrium_field(lbm_ctx* c, const double* rho, const double* u) {
+  if (!c || !rho || !u) return fail(LBM_ERR_INVALID_ARG, "null argument");
+  const int st = c->storage;
+  const size_t N = c->N;
+  std::vector<double> h(size_t(c->Q) * N), r(N), uv(size_t(c->D) * N);
+  double eq[27], uc[3] = {0.0, 0.0, 0.0};
+  for (size_t x = 0; x < N; ++x) {
+    for (int d = 0; d < c->D; ++d) uc[d] = u[size_t(d) * N + x];
+    host_equilibrium(c->lattice, st, rho[x], uc, eq);
+    for (int i = 0; i < c->Q; ++i) h[size_t(i) * N + x] = eq[i];
+    r[x] = to_T(st, rho[x]);
+    for (int d = 0; d < c->D; ++d) uv[size_t(d) * N + x] = to_T(st, uc[d]);
+  }
+  lbm_status s = lbm_set_populations(c, h.data());
+  if (s) return s;
   return lbm_set_macros(c, r.data(), uv.data());
 }
 
@@ -1059,7 +1097,10 @@ lbm_status lbm_batch_run(lbm_batch* b, uint64_t t0, double tol, uint64_t check_e
     for (uint32_t m = 0; m < M; ++m) {
       if (!b->active[m]) continue;
       steps[m] += n;
-      const double r = sums[2 * m + 1] > 0 ? std::sqrt(sums[2 * m] / sums[2 * m + 1]) : 0.0;
+      // a diverged member (non-finite sums) reports NaN, never "converged"
+      const double num = sums[2 * m], den = sums[2 * m + 1];
+      const double r = !(std::isfinite(num) && std::isfinite(den)) ? std::nan("")
+                       : (den > 0 ? std::sqrt(num / den) : 0.0);
       res[m] = r;
       if (!(r >= tol)) {  // converged (or NaN: stop and report it)
         b->active[m] = 0;

@@ -85,3 +85,25 @@ def test_batch_autotune_restores_state(hip):
     b.step(10)
     ms_, upd = b.stats()
     assert upd == 10 * sum(int(np.prod(m[0])) for m in ms) and ms_ > 0
+
+
+def test_batch_run_reports_diverged_member_as_nan(hip):
+    """A member holding a non-finite population stops with residual NaN and
+    is never reported converged; the other members are unaffected."""
+    from lbm_mirror_amd.lattice import Batch
+    rng = np.random.default_rng(9)
+    ms_ = [c4_member(rng, side_lo=32, side_hi=40) for _ in range(3)]
+    b = Batch(abi.D2Q9, abi.F32, [m[0] for m in ms_])
+    for mem, m in zip(b.members, ms_):
+        ext, flags, ops, bcu, _ = m
+        mem.set_flags(flags)
+        for fl, op in ops.items():
+            mem.set_operator(fl, op)
+        mem.set_boundary_velocity(bcu)
+        mem.init_equilibrium(1.0, (0.0, 0.0))
+    f = b.members[1].get_populations()
+    f[:, f.shape[1] // 2] = np.nan
+    b.members[1].set_populations(f)
+    steps, res = b.run(tol=1e-12, check_every=4, max_steps=8)
+    assert np.isnan(res[1]) and steps[1] == 4
+    assert np.all(np.isfinite(res[[0, 2]])) and np.all(steps[[0, 2]] == 8)
