@@ -35,7 +35,7 @@ namespace lbmx {
 #define LBMX_VEC_ODD 1
 #endif
 #ifndef LBMX_ROW_MIN_NX
-#define LBMX_ROW_MIN_NX 64
+#define LBMX_ROW_MIN_NX (1 << 30)
 #endif
 constexpr int kVecF32 = LBMX_VEC_F32;
 constexpr int kRowKernelMinNx = LBMX_ROW_MIN_NX;
