@@ -110,43 +110,54 @@ def bench_c1(args):
 def bench_c3(args, storage):
     """Config 3: D2Q9 PSM particle-laden channel 2048x512, 200 discs r~U[8,16]
     placed by seeded rejection, density 1.5, gravity 1e-6, DEM k=1e3,
-    restitution 0.9 (zeta from e), tau 0.7; full particle pipeline each step."""
+    restitution 0.9 (zeta from e), tau 0.7; full particle pipeline each step.
+    The same run in FP64 gives the error of the timed storage against FP64
+    (velocity field and particle trajectories)."""
     import numpy as np
     from lbm_mirror_amd import abi
     from lbm_mirror_amd.lattice import Lattice, spheroid_mass
     from helpers import channel_flags_2d, seeded_discs, restitution_to_zeta
     nx, ny = 2048, 512
-    lat = Lattice(abi.D2Q9, storage, (nx, ny))
-    lat.set_flags(channel_flags_2d(nx, ny))
-    ops = {1: abi.op_desc(boundary=abi.BC_BOUNCE_BACK),
-           2: abi.op_desc(collision=abi.COLL_PSM, tau=0.7),
-           3: abi.op_desc(boundary=abi.BC_ZOUHE_PRESSURE_X, collision=abi.COLL_BGK, east=1, tau=0.7,
-                          ramp=(1.0, 0.001 / 64, 1.001, 64)),
-           4: abi.op_desc(boundary=abi.BC_ZOUHE_PRESSURE_X, collision=abi.COLL_BGK, east=0,
-                          tau=0.7, rho=1.0)}
-    for fl, op in ops.items():
-        lat.set_operator(fl, op)
-    lat.init_equilibrium(1.0, (0.0, 0.0))
     discs = seeded_discs(200, (nx, ny), 8.0, 16.0, seed=20260103)
-    parts = []
-    for (x, y, r) in discs:
-        p = abi.Particle()
-        p.position[0] = p.position_old[0] = x
-        p.position[1] = p.position_old[1] = y
-        p.radius, p.density = r, 1.5
-        p.mass, p.inertia = spheroid_mass(2, storage if storage in (abi.F64, abi.F32) else abi.F32, r, 1.5)
-        parts.append(p)
-    prm = abi.particle_params(k_n=1e3, zeta_n=restitution_to_zeta(0.9), walls=True, sub_steps=16,
-                              body_acceleration=(0.0, -1e-6), eps=1.5, max_radius=16.0)
-    lat.particles_set(parts, prm)
     steps = args.steps if args.steps != 1000 else 2000
-    lat.step(args.warmup, t0=0)
-    lat.sync()
-    lat.step(steps, t0=args.warmup)
-    lat.sync()
-    ms = lat.last_step_ms()
-    ps = lat.particles_get()
-    finite = all(np.isfinite(q.position[0]) and np.isfinite(q.position[1]) for q in ps)
+
+    def run(st):
+        lat = Lattice(abi.D2Q9, st, (nx, ny))
+        lat.set_flags(channel_flags_2d(nx, ny))
+        ops = {1: abi.op_desc(boundary=abi.BC_BOUNCE_BACK),
+               2: abi.op_desc(collision=abi.COLL_PSM, tau=0.7),
+               3: abi.op_desc(boundary=abi.BC_ZOUHE_PRESSURE_X, collision=abi.COLL_BGK, east=1, tau=0.7,
+                              ramp=(1.0, 0.001 / 64, 1.001, 64)),
+               4: abi.op_desc(boundary=abi.BC_ZOUHE_PRESSURE_X, collision=abi.COLL_BGK, east=0,
+                              tau=0.7, rho=1.0)}
+        for fl, op in ops.items():
+            lat.set_operator(fl, op)
+        lat.init_equilibrium(1.0, (0.0, 0.0))
+        parts = []
+        for (x, y, r) in discs:
+            p = abi.Particle()
+            p.position[0] = p.position_old[0] = x
+            p.position[1] = p.position_old[1] = y
+            p.radius, p.density = r, 1.5
+            p.mass, p.inertia = spheroid_mass(2, abi.F64 if st == abi.F64 else abi.F32, r, 1.5)
+            parts.append(p)
+        prm = abi.particle_params(k_n=1e3, zeta_n=restitution_to_zeta(0.9), walls=True, sub_steps=16,
+                                  body_acceleration=(0.0, -1e-6), eps=1.5, max_radius=16.0)
+        lat.particles_set(parts, prm)
+        lat.step(args.warmup, t0=0)
+        lat.sync()
+        lat.step(steps, t0=args.warmup)
+        lat.sync()
+        ms = lat.last_step_ms()
+        pos = np.array([[q.position[0], q.position[1]] for q in lat.particles_get()])
+        return ms, lat.get_macros()[1], pos
+
+    ms, u, pos = run(storage)
+    _, u64, pos64 = run(abi.F64)
+    x0 = np.array([[x, y] for (x, y, _) in discs])
+    disp = float(np.max(np.linalg.norm(pos64 - x0, axis=1)))
+    err_u = float(np.linalg.norm(u - u64) / np.linalg.norm(u64))
+    err_x = float(np.max(np.linalg.norm(pos - pos64, axis=1)))
     sb = {abi.F32: 4, abi.F16S: 2, abi.BF16S: 2}[storage]
     return {"metric": "MLUPS (D2Q9 PSM particle-laden channel, full step incl. particles)",
             "value": round(nx * ny * steps / (ms * 1e3), 1), "unit": "MLUPS", "n_gpus": 1,
@@ -155,7 +166,9 @@ def bench_c3(args, storage):
             "dtype": {abi.F32: "f32", abi.F16S: "f16-storage/f32", abi.BF16S: "bf16-storage/f32"}[storage],
             "data": "synthetic",
             "config": {"workload": "D2Q9 PSM 2048x512, 200 discs r~U[8,16], rho_p 1.5, g 1e-6, "
-                                   "DEM k 1e3 e 0.9, tau 0.7", "particles_finite": bool(finite)},
+                                   "DEM k 1e3 e 0.9, tau 0.7",
+                       "error_vs_fp64": {"velocity_rel_l2": err_u, "trajectory_max_cells": err_x,
+                                         "max_displacement_cells": disp}},
             "roofline": {"bound": "hbm",
                          "achieved": round(nx * ny * 2 * 9 * sb / (ms / steps * 1e-3) / 1e9, 1),
                          "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -301,6 +301,12 @@ lbm_status lbm_net_forward_raw(lbm_net* net, uint32_t count, const float* x, flo
  * t[count*N*3], mask m[count*N] (1 = counted); grad has param_count floats. */
 lbm_status lbm_net_loss_grad_raw(lbm_net* net, uint32_t count, const float* x, const float* t,
                                  const float* m, double* loss, float* grad);
+/* Output standardisation: the network's outputs are (target - mean) / stdev
+ * per channel (ux, uy, rho targets above).  The first lbm_net_train call takes
+ * mean and stdev from the training set's fluid cells and freezes them; a
+ * network restored with lbm_net_set_params needs them restored too. */
+lbm_status lbm_net_get_output_scale(lbm_net* net, float mean[3], float stdev[3]);
+lbm_status lbm_net_set_output_scale(lbm_net* net, const float mean[3], const float stdev[3]);
 /* Device time of the last train / predict call. */
 lbm_status lbm_net_last_ms(lbm_net* net, double* ms);
 

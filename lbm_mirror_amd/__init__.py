@@ -40,6 +40,7 @@ EXPORTS = [
     "lbm_net_get_params", "lbm_net_set_params", "lbm_net_features", "lbm_net_add_sample",
     "lbm_net_clear_samples", "lbm_net_sample_count", "lbm_net_train", "lbm_net_predict",
     "lbm_net_forward_raw", "lbm_net_loss_grad_raw", "lbm_net_last_ms",
+    "lbm_net_get_output_scale", "lbm_net_set_output_scale",
 ]
 
 
@@ -111,6 +112,8 @@ def load_library(path=None):
     L.lbm_net_forward_raw.argtypes = [vp, C.c_uint32, fp, fp]
     L.lbm_net_loss_grad_raw.argtypes = [vp, C.c_uint32, fp, fp, fp, dp, fp]
     L.lbm_net_last_ms.argtypes = [vp, dp]
+    L.lbm_net_get_output_scale.argtypes = [vp, fp, fp]
+    L.lbm_net_set_output_scale.argtypes = [vp, fp, fp]
     for name in EXPORTS:
         f = getattr(L, name)
         if name not in ("lbm_last_error", "lbm_version", "lbm_device_count", "lbm_batch_member"):

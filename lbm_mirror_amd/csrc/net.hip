@@ -420,6 +420,16 @@ __global__ void k_wconvert(const float* wm, int cin, int cr, int cpad, uint16_t*
   wt[(size_t(8 - tap) * cpad + co) * cin + ci] = v;
 }
 
+// In-place standardisation of stored targets: t = (t - mu) / sd.
+__global__ void k_standardize(float* t, size_t n, float m0, float m1, float m2, float s0, float s1, float s2) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int c = int(i % 3);
+  const float m = c == 0 ? m0 : (c == 1 ? m1 : m2);
+  const float sd = c == 0 ? s0 : (c == 1 ? s1 : s2);
+  t[i] = (t[i] - m) / sd;
+}
+
 // Minibatch assembly from the on-device training set.
 __global__ void k_gather(const uint16_t* xs, const float* ts, const uint8_t* ms, const uint32_t* idx, int B, int HW,
                          uint16_t* x, float* t, uint8_t* m) {
@@ -506,6 +516,11 @@ struct lbm_net {
   float* ts = nullptr;
   uint8_t* ms = nullptr;
   std::vector<double> nf;  // counted pixels per sample
+  // output standardisation: the network predicts (target - mu) / sd per
+  // channel; mu, sd come from the training set at the first training call
+  double tsum[3] = {0, 0, 0}, tsq[3] = {0, 0, 0}, tcnt = 0;
+  bool frozen = false;
+  float mu[3] = {0.f, 0.f, 0.f}, sd[3] = {1.f, 1.f, 1.f};
 };
 
 namespace {
@@ -989,7 +1004,15 @@ lbm_status lbm_net_add_sample(lbm_net* n, const uint8_t* flags, double u_max, do
     t[i * 3 + 0] = float(u[i] / u_max);
     t[i * 3 + 1] = float(u[HW + i] / u_max);
     t[i * 3 + 2] = float((rho[i] - 1.0) / ru);
+    if (m[i])
+      for (int c = 0; c < 3; ++c) {
+        n->tsum[c] += t[i * 3 + c];
+        n->tsq[c] += double(t[i * 3 + c]) * t[i * 3 + c];
+      }
   }
+  n->tcnt += cnt;
+  if (n->frozen)
+    for (size_t i = 0; i < HW * 3; ++i) t[i] = (t[i] - n->mu[i % 3]) / n->sd[i % 3];
   NET_TRY(hipSetDevice(n->device));
   NET_TRY(hipMemcpyAsync(n->xs + k * HW * 8, xb.data(), HW * 8 * sizeof(uint16_t), hipMemcpyHostToDevice, n->stream));
   NET_TRY(hipMemcpyAsync(n->ts + k * HW * 3, t.data(), HW * 3 * sizeof(float), hipMemcpyHostToDevice, n->stream));
@@ -1018,6 +1041,19 @@ lbm_status lbm_net_train(lbm_net* n, uint32_t steps, float lr, uint64_t seed, fl
   if (S < uint32_t(B)) return nfail(LBM_ERR_INVALID_STATE, "training set holds %u samples, batch is %d", S, B);
   if (steps == 0) return LBM_OK;
   NET_TRY(hipSetDevice(n->device));
+  if (!n->frozen && n->tcnt > 0) {
+    for (int c = 0; c < 3; ++c) {
+      const double m = n->tsum[c] / n->tcnt;
+      const double v = std::max(n->tsq[c] / n->tcnt - m * m, 0.0);
+      n->mu[c] = float(m);
+      n->sd[c] = float(std::max(std::sqrt(v), 1e-6));
+    }
+    const size_t cnt = size_t(S) * n->cfg.height * n->cfg.width * 3;
+    hipLaunchKernelGGL(k_standardize, dim3(blocks_for(cnt)), dim3(256), 0, n->stream, n->ts, cnt, n->mu[0], n->mu[1],
+                       n->mu[2], n->sd[0], n->sd[1], n->sd[2]);
+    NET_TRY(hipGetLastError());
+    n->frozen = true;
+  }
   // minibatch indices: a fresh seeded permutation per epoch
   std::vector<uint32_t> idx(size_t(steps) * B), perm(S);
   std::mt19937_64 rng(seed);
@@ -1110,9 +1146,9 @@ lbm_status lbm_net_predict(lbm_net* n, uint32_t count, const uint8_t* flags, con
       for (size_t i = 0; i < HW; ++i) {
         const float* o = yf.data() + (b * HW + i) * 3;
         if (fl[i] >= 2) {
-          ux[i] = double(o[0]) * u_max[k];
-          uy[i] = double(o[1]) * u_max[k];
-          r[i] = 1.0 + double(o[2]) * rho_unit(u_max[k]);
+          ux[i] = (double(o[0]) * n->sd[0] + n->mu[0]) * u_max[k];
+          uy[i] = (double(o[1]) * n->sd[1] + n->mu[1]) * u_max[k];
+          r[i] = 1.0 + (double(o[2]) * n->sd[2] + n->mu[2]) * rho_unit(u_max[k]);
         } else {
           ux[i] = 0.0;
           uy[i] = 0.0;
@@ -1149,6 +1185,28 @@ lbm_status lbm_net_loss_grad_raw(lbm_net* n, uint32_t count, const float* x, con
   NET_TRY(hipMemcpyAsync(loss, n->dloss, sizeof(double), hipMemcpyDeviceToHost, n->stream));
   NET_TRY(hipMemcpyAsync(grad, n->g, n->nparam * sizeof(float), hipMemcpyDeviceToHost, n->stream));
   NET_TRY(hipStreamSynchronize(n->stream));
+  return LBM_OK;
+}
+
+lbm_status lbm_net_get_output_scale(lbm_net* n, float mean[3], float stdev[3]) {
+  if (!n || !mean || !stdev) return nfail(LBM_ERR_INVALID_ARG, "null argument");
+  for (int c = 0; c < 3; ++c) {
+    mean[c] = n->mu[c];
+    stdev[c] = n->sd[c];
+  }
+  return LBM_OK;
+}
+
+lbm_status lbm_net_set_output_scale(lbm_net* n, const float mean[3], const float stdev[3]) {
+  if (!n || !mean || !stdev) return nfail(LBM_ERR_INVALID_ARG, "null argument");
+  if (n->frozen && !n->nf.empty())
+    return nfail(LBM_ERR_INVALID_STATE, "training set already standardised with other statistics");
+  for (int c = 0; c < 3; ++c) {
+    if (!(stdev[c] > 0.f)) return nfail(LBM_ERR_INVALID_ARG, "stdev must be positive");
+    n->mu[c] = mean[c];
+    n->sd[c] = stdev[c];
+  }
+  n->frozen = true;
   return LBM_OK;
 }
 

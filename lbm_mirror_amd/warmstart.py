@@ -118,6 +118,17 @@ class Net:
         check(self.L.lbm_net_loss_grad_raw(self._h, k, _fp(x), _fp(t), _fp(m), C.byref(loss), _fp(g)))
         return loss.value, g
 
+    def output_scale(self):
+        m = np.zeros(3, dtype=np.float32)
+        s = np.zeros(3, dtype=np.float32)
+        check(self.L.lbm_net_get_output_scale(self._h, _fp(m), _fp(s)))
+        return m, s
+
+    def set_output_scale(self, mean, stdev):
+        m = np.ascontiguousarray(mean, dtype=np.float32)
+        s = np.ascontiguousarray(stdev, dtype=np.float32)
+        check(self.L.lbm_net_set_output_scale(self._h, _fp(m), _fp(s)))
+
     def last_ms(self):
         m = C.c_double()
         check(self.L.lbm_net_last_ms(self._h, C.byref(m)))

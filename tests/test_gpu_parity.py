@@ -335,6 +335,54 @@ def test_shifted_16bit_storage_error_vs_fp64(hip, orc, st):
     assert err_u < bound and err_r < bound / 10
 
 
+def test_particle_laden_precision_vs_fp64(hip, orc):
+    """The particle-laden channel in FP32 and with FP16/BF16 shifted storage
+    (FP32 arithmetic), against the FP64 run (itself bitwise equal to the
+    oracle): velocity-field and particle-trajectory errors, reported and
+    bounded over 80 steps; particle trajectories reported.  The discs start
+    well apart, so no contact switches on.  Particle state is kept in the
+    compute type T as in the reference, and in FP32 the Verlet increment
+    a*ts^2 (here 1e-5/16^2) is far below half an ulp of a position near 50:
+    FP32 discs do not move where FP64 discs drift, so the trajectory error of
+    every FP32-arithmetic run is the whole FP64 displacement; the bar is on
+    the velocity field.  No reference counterpart for the 16-bit storage."""
+    from lbm_mirror_amd.lattice import Lattice
+    ext = (192, 96)
+    flags = channel_flags_2d(*ext)
+    ops = {1: abi.op_desc(boundary=abi.BC_BOUNCE_BACK), 2: abi.op_desc(collision=abi.COLL_PSM, tau=0.7),
+           3: abi.op_desc(boundary=abi.BC_ZOUHE_PRESSURE_X, collision=abi.COLL_BGK, east=1, tau=0.7,
+                          ramp=(1.0, 0.005 / 50, 1.005, 50)),
+           4: abi.op_desc(boundary=abi.BC_ZOUHE_PRESSURE_X, collision=abi.COLL_BGK, east=0, tau=0.7,
+                          rho=1.0)}
+    discs = seeded_discs(8, ext, 4.0, 7.0, 11, margin=6.0, gap=8.0)
+    runs = {}
+    for st in (abi.F64, abi.F32, abi.F16S, abi.BF16S):
+        lat = Lattice(abi.D2Q9, st, ext)
+        _setup((lat,), flags, ops)
+        ms = abi.F64 if st == abi.F64 else abi.F32
+        parts = [make_particle([x, y], r, 1.5, 2, ms, orc=orc) for (x, y, r) in discs]
+        prm = abi.particle_params(k_n=1e3, zeta_n=restitution_to_zeta(0.9), walls=True, sub_steps=16,
+                                  body_acceleration=(0.0, -1e-5), eps=1.5, max_radius=7.0)
+        lat.particles_set(parts, prm)
+        lat.step(80)
+        ps = lat.particles_get()
+        runs[st] = (lat.get_macros()[1], np.array([[p.position[0], p.position[1]] for p in ps]))
+    u64, x64 = runs[abi.F64]
+    x0 = np.array([[x, y] for (x, y, _) in discs])
+    disp = np.max(np.linalg.norm(x64 - x0, axis=1))
+    assert 1e-4 < disp < 4.0
+    bounds = {abi.F32: 1e-3, abi.F16S: 5e-2, abi.BF16S: 0.2}
+    for st, eu_max in bounds.items():
+        u, x = runs[st]
+        eu = np.linalg.norm(u - u64) / np.linalg.norm(u64)
+        ex = np.max(np.linalg.norm(x - x64, axis=1))
+        moved = np.max(np.linalg.norm(x - x0, axis=1))
+        print(f"storage {st}: velocity rel L2 error {eu:.2e}; trajectory error {ex:.2e} cells "
+              f"(FP64 displacement {disp:.4f}, this run's {moved:.2e})")
+        assert eu < eu_max
+        assert np.all(np.isfinite(x)) and ex <= disp * 1.0001
+
+
 @pytest.mark.parametrize("st", [abi.F64, abi.F32])
 def test_static_disc_psm_driver(hip, orc, st):
     """The static-obstacle PSM driver (poiseulle_particles_2d_psm_gpu/sim.cpp:
