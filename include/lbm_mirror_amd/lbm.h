@@ -131,6 +131,11 @@ typedef struct lbm_particle_params {
   double body_acceleration[3];      /* ParticleIntegrate body acceleration           */
   double eps;                       /* porosity ramp width (psm.hpp:213: 1.5)        */
   double max_radius;                /* cell-list sizing (cell_list.hpp:185-206)      */
+  int32_t state_fp64;               /* 0: particle state and arithmetic in the lattice's
+                                       compute type, as the reference (default);
+                                       1: FP64 particle state and arithmetic whatever
+                                       the storage (no reference counterpart)       */
+  int32_t reserved;
 } lbm_particle_params;
 
 typedef struct lbm_ctx lbm_ctx;

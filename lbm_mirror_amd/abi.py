@@ -92,6 +92,8 @@ class ParticleParams(C.Structure):
         ("body_acceleration", C.c_double * 3),
         ("eps", C.c_double),
         ("max_radius", C.c_double),
+        ("state_fp64", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 
@@ -118,7 +120,7 @@ def op_desc(boundary=BC_NONE, collision=COLL_NONE, east=0, flags=0, tau=1.0,
 def particle_params(k_n=1.0, zeta_n=0.3, gamma_t=0.0, mu=0.0, walls=False,
                     porosity_mode=POROSITY_SCATTER, psm_torque=False,
                     cell_capacity=4, sub_steps=1, body_acceleration=(0.0, 0.0, 0.0),
-                    eps=1.5, max_radius=1.0):
+                    eps=1.5, max_radius=1.0, state_fp64=False):
     p = ParticleParams()
     p.k_n, p.zeta_n, p.gamma_t, p.mu = k_n, zeta_n, gamma_t, mu
     p.walls, p.porosity_mode, p.psm_torque = int(walls), porosity_mode, int(psm_torque)
@@ -126,6 +128,7 @@ def particle_params(k_n=1.0, zeta_n=0.3, gamma_t=0.0, mu=0.0, walls=False,
     for i in range(3):
         p.body_acceleration[i] = body_acceleration[i] if i < len(body_acceleration) else 0.0
     p.eps, p.max_radius = eps, max_radius
+    p.state_fp64 = int(state_fp64)
     return p
 
 

@@ -103,6 +103,7 @@ struct ParticleArgs {
   double bin;       // bin size (T-rounded)
   uint64_t grid[3];
   uint32_t cap;
+  int32_t state64;  // particle records and arithmetic in FP64 whatever the storage
 };
 
 // Per-particle record layout (T values)

@@ -84,6 +84,7 @@ struct lbm_ctx {
   LaunchShape shape;
   // particles
   uint32_t np = 0;
+  bool pdbl = false;  // particle records are doubles
   void* parts = nullptr;
   double* f64 = nullptr;
   double* t64 = nullptr;
@@ -135,30 +136,37 @@ Geometry geom(const lbm_ctx* c) {
   return g;
 }
 
-lbm_status put_field(lbm_ctx* c, void* field, const double* host, size_t count) {
+lbm_status put_field(lbm_ctx* c, void* field, const double* host, size_t count, int as = -1) {
   double* tmp = nullptr;
   HIP_TRY(hipSetDevice(c->device));
   HIP_TRY(hipMalloc(&tmp, count * sizeof(double)));
   HIP_TRY(hipMemcpyAsync(tmp, host, count * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  HIP_TRY(field_from_double(compute_double(c->storage), field, tmp, count, c->stream));
+  HIP_TRY(field_from_double(compute_double(as < 0 ? c->storage : as), field, tmp, count, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   HIP_TRY(hipFree(tmp));
   return LBM_OK;
 }
 
-lbm_status get_field(lbm_ctx* c, const void* field, double* host, size_t count) {
+lbm_status get_field(lbm_ctx* c, const void* field, double* host, size_t count, int as = -1) {
   double* tmp = nullptr;
   HIP_TRY(hipSetDevice(c->device));
   HIP_TRY(hipMalloc(&tmp, count * sizeof(double)));
-  HIP_TRY(field_to_double(compute_double(c->storage), field, tmp, count, c->stream));
+  HIP_TRY(field_to_double(compute_double(as < 0 ? c->storage : as), field, tmp, count, c->stream));
   HIP_TRY(hipMemcpyAsync(host, tmp, count * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   HIP_TRY(hipFree(tmp));
   return LBM_OK;
 }
 
+// Storage whose compute type the particle records and arithmetic use.
+int particle_storage(int storage, const lbm_particle_params& pp) {
+  return (storage == LBM_F64 || pp.state_fp64) ? LBM_F64 : LBM_F32;
+}
+
 ParticleArgs particle_args(lbm_ctx* c) {
   ParticleArgs a{};
+  const int pst = particle_storage(c->storage, c->pp);
+  a.state64 = pst == LBM_F64 ? 1 : 0;
   a.f = c->f;
   a.por = c->por;
   a.force = c->force;
@@ -175,7 +183,7 @@ ParticleArgs particle_args(lbm_ctx* c) {
   a.t64 = c->t64;
   a.partial = c->partial;
   a.blocks_per_particle = c->bpp;
-  a.eps = to_T(c->storage, c->pp.eps);
+  a.eps = to_T(pst, c->pp.eps);
   a.sub_steps = c->pp.sub_steps;
   a.porosity_mode = c->pp.porosity_mode;
   a.psm_torque = c->pp.psm_torque;
@@ -184,7 +192,7 @@ ParticleArgs particle_args(lbm_ctx* c) {
   a.gamma_t = c->pp.gamma_t;
   a.mu = c->pp.mu;
   a.walls = c->pp.walls;
-  for (int d = 0; d < 3; ++d) a.g[d] = to_T(c->storage, c->pp.body_acceleration[d]);
+  for (int d = 0; d < 3; ++d) a.g[d] = to_T(pst, c->pp.body_acceleration[d]);
   a.cl_count = c->cl_count;
   a.cl_slot = c->cl_slot;
   a.cl_overflow = c->cl_overflow;
@@ -628,10 +636,10 @@ lbm_status lbm_particles_set(lbm_ctx* c, const lbm_particle* p, uint32_t n,
     return fail(LBM_ERR_INVALID_ARG, "cell capacity must be positive");
   HIP_TRY(hipSetDevice(c->device));
   HIP_TRY(hipStreamSynchronize(c->stream));
-  const int st = c->storage;
+  const int st = particle_storage(c->storage, *prm);
   const bool dbl = compute_double(st);
   // (re)allocate
-  if (n != c->np) {
+  if (n != c->np || dbl != c->pdbl) {
     void* ptrs[] = {c->parts, c->f64, c->t64, c->partial};
     for (void* q : ptrs)
       if (q) HIP_TRY(hipFree(q));
@@ -639,7 +647,7 @@ lbm_status lbm_particles_set(lbm_ctx* c, const lbm_particle* p, uint32_t n,
     c->f64 = c->t64 = c->partial = nullptr;
     c->np = 0;
     if (n > 0) {
-      HIP_TRY(alloc_zero(&c->parts, size_t(n) * kPartFields * c->tsz(), c->stream));
+      HIP_TRY(alloc_zero(&c->parts, size_t(n) * kPartFields * (dbl ? 8 : 4), c->stream));
       HIP_TRY(alloc_zero((void**)&c->f64, size_t(n) * 3 * sizeof(double), c->stream));
       HIP_TRY(alloc_zero((void**)&c->t64, size_t(n) * 3 * sizeof(double), c->stream));
     }
@@ -710,10 +718,12 @@ lbm_status lbm_particles_set(lbm_ctx* c, const lbm_particle* p, uint32_t n,
       r[PF_RADIUS] = to_T(st, p[k].radius);
     }
     c->np = n;
-    lbm_status s = put_field(c, c->parts, rec.data(), rec.size());
+    c->pdbl = dbl;
+    lbm_status s = put_field(c, c->parts, rec.data(), rec.size(), st);
     if (s) return s;
   }
   c->np = n;
+  c->pdbl = dbl;
   return LBM_OK;
 }
 
@@ -722,7 +732,7 @@ lbm_status lbm_particles_get(lbm_ctx* c, lbm_particle* p, uint32_t n) {
   if (n > c->np) return fail(LBM_ERR_INVALID_ARG, "asked for %u of %u particles", n, c->np);
   if (n == 0) return LBM_OK;
   std::vector<double> rec(size_t(c->np) * kPartFields);
-  lbm_status s = get_field(c, c->parts, rec.data(), rec.size());
+  lbm_status s = get_field(c, c->parts, rec.data(), rec.size(), c->pdbl ? LBM_F64 : LBM_F32);
   if (s) return s;
   for (uint32_t k = 0; k < n; ++k) {
     const double* r = rec.data() + size_t(k) * kPartFields;

@@ -34,6 +34,17 @@ template <>
 __device__ inline float ldp<__hip_bfloat16, float>(const __hip_bfloat16* p, float w) {
   return __bfloat162float(*p) + w;
 }
+// FP64 particle state over FP32-arithmetic storage: the FP32 value, widened
+template <>
+__device__ inline double ldp<float, double>(const float* p, double) { return double(*p); }
+template <>
+__device__ inline double ldp<__half, double>(const __half* p, double w) {
+  return double(ldp<__half, float>(p, float(w)));
+}
+template <>
+__device__ inline double ldp<__hip_bfloat16, double>(const __hip_bfloat16* p, double w) {
+  return double(ldp<__hip_bfloat16, float>(p, float(w)));
+}
 
 // wrap a coordinate along a periodic axis; false when it leaves the lattice
 __device__ inline bool wrap(int& v, int n, bool periodic) {
@@ -148,7 +159,8 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
-template <class L, class S, class T>
+// S: population storage, T: particle arithmetic, F: fluid field type
+template <class L, class S, class T, class F>
 __global__ __launch_bounds__(256) void k_gather(ParticleArgs a) {
   constexpr int D = L::D, Q = L::Q;
   const uint32_t p = blockIdx.x / a.blocks_per_particle;
@@ -162,8 +174,8 @@ __global__ __launch_bounds__(256) void k_gather(ParticleArgs a) {
   const T ts = one / T(a.sub_steps);
   T vp[3] = {T(0), T(0), T(0)};
   for (int d = 0; d < D; ++d) vp[d] = (P[PF_POS + d] - P[PF_POS_OLD + d]) / ts;
-  T* por = static_cast<T*>(a.por);
-  T* fout = static_cast<T*>(a.force);
+  F* por = static_cast<F*>(a.por);
+  F* fout = static_cast<F*>(a.force);
   const bool scatter = a.porosity_mode == LBM_POROSITY_SCATTER;
   double acc[6] = {0, 0, 0, 0, 0, 0};
   for (uint32_t k = chunk * blockDim.x + threadIdx.x; k < b.count;
@@ -173,7 +185,7 @@ __global__ __launch_bounds__(256) void k_gather(ParticleArgs a) {
     T rel[3] = {T(0), T(0), T(0)};
     const T B = porosity_at<T, D>(P, x, eps, rel);
     const uint32_t c = uint32_t(x[0]) + uint32_t(a.nx) * (uint32_t(x[1]) + uint32_t(a.ny) * uint32_t(x[2]));
-    if (scatter && a.owner[2 * c] == p + 1) por[c] = B;
+    if (scatter && a.owner[2 * c] == p + 1) por[c] = F(B);
     if (B >= T(1.0)) continue;
     T mom[3] = {T(0.0), T(0.0), T(0.0)};
 #pragma unroll
@@ -195,7 +207,7 @@ __global__ __launch_bounds__(256) void k_gather(ParticleArgs a) {
     T fc[3] = {T(0), T(0), T(0)};
     for (int d = 0; d < D; ++d) fc[d] = mom[d] * flip;
     if (a.owner[2 * c + 1] == p + 1)
-      for (int d = 0; d < D; ++d) fout[size_t(d) * a.N + c] = fc[d];
+      for (int d = 0; d < D; ++d) fout[size_t(d) * a.N + c] = F(fc[d]);
     for (int d = 0; d < D; ++d) acc[d] -= double(fc[d]);
     if (D == 2) {
       acc[3] -= double(rel[0] * fc[1] - rel[1] * fc[0]);
@@ -278,7 +290,7 @@ __global__ __launch_bounds__(1024) void k_build_cells(ParticleArgs a, uint64_t b
 }
 
 // cell_porosity (couple.hpp:26-60) with for_each_bin_candidate's order
-template <class L, class T>
+template <class L, class T, class F>
 __global__ __launch_bounds__(256) void k_product(ParticleArgs a) {
   constexpr int D = L::D;
   const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -316,7 +328,7 @@ __global__ __launch_bounds__(256) void k_product(ParticleArgs a) {
       B = B * disc_porosity<T, D>(rel, P[PF_RADIUS], eps);
     }
   }
-  static_cast<T*>(a.por)[c] = B;
+  static_cast<F*>(a.por)[c] = F(B);
 }
 
 // ---------------------------------------------------------------- DEM
@@ -515,32 +527,41 @@ __global__ void k_integrate(ParticleArgs a) {
 }
 
 // ---------------------------------------------------------------- launchers
-template <class L, class S, class T>
+template <class L, class S, class T, class F>
 static hipError_t couple_t(const ParticleArgs& a, hipStream_t s) {
-  const bool dbl = sizeof(T) == 8;
+  const bool fdbl = sizeof(F) == 8;
   const uint32_t N = a.N;
   const bool scatter = a.porosity_mode == LBM_POROSITY_SCATTER;
-  hipLaunchKernelGGL(k_clear, dim3((N + 255) / 256), dim3(256), 0, s, dbl ? nullptr : (float*)a.por,
-                     dbl ? (double*)a.por : nullptr, a.owner, N, scatter ? 1 : 0);
+  hipLaunchKernelGGL(k_clear, dim3((N + 255) / 256), dim3(256), 0, s, fdbl ? nullptr : (float*)a.por,
+                     fdbl ? (double*)a.por : nullptr, a.owner, N, scatter ? 1 : 0);
   const dim3 pg(a.n * a.blocks_per_particle);
   hipLaunchKernelGGL((k_owner<L, T>), pg, dim3(256), 0, s, a);
-  hipLaunchKernelGGL((k_gather<L, S, T>), pg, dim3(256), 0, s, a);
+  hipLaunchKernelGGL((k_gather<L, S, T, F>), pg, dim3(256), 0, s, a);
   hipLaunchKernelGGL((k_finalize<T, L::D>), dim3((a.n + 255) / 256), dim3(256), 0, s, a);
   if (!scatter) {
     uint64_t bins = a.grid[0] * a.grid[1] * a.grid[2];
     hipLaunchKernelGGL((k_build_cells<T, L::D>), dim3(1), dim3(1024), 0, s, a, bins);
-    hipLaunchKernelGGL((k_product<L, T>), dim3((N + 255) / 256), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((k_product<L, T, F>), dim3((N + 255) / 256), dim3(256), 0, s, a);
   }
   return hipGetLastError();
 }
 
 template <class L>
 static hipError_t couple_l(int storage, const ParticleArgs& a, hipStream_t s) {
+  if (a.state64) {
+    switch (storage) {
+      case LBM_F64: return couple_t<L, double, double, double>(a, s);
+      case LBM_F32: return couple_t<L, float, double, float>(a, s);
+      case LBM_F16S: return couple_t<L, __half, double, float>(a, s);
+      case LBM_BF16S: return couple_t<L, __hip_bfloat16, double, float>(a, s);
+    }
+    return hipErrorInvalidValue;
+  }
   switch (storage) {
-    case LBM_F64: return couple_t<L, double, double>(a, s);
-    case LBM_F32: return couple_t<L, float, float>(a, s);
-    case LBM_F16S: return couple_t<L, __half, float>(a, s);
-    case LBM_BF16S: return couple_t<L, __hip_bfloat16, float>(a, s);
+    case LBM_F64: return couple_t<L, double, double, double>(a, s);
+    case LBM_F32: return couple_t<L, float, float, float>(a, s);
+    case LBM_F16S: return couple_t<L, __half, float, float>(a, s);
+    case LBM_BF16S: return couple_t<L, __hip_bfloat16, float, float>(a, s);
   }
   return hipErrorInvalidValue;
 }
@@ -558,7 +579,7 @@ hipError_t launch_psm_couple(int lattice, int storage, const ParticleArgs& a, hi
 hipError_t launch_dem_contact(int dims, int storage, const ParticleArgs& a, hipStream_t s) {
   if (a.n == 0) return hipSuccess;
   const dim3 g((a.n + 255) / 256), b(256);
-  const bool dbl = storage == LBM_F64;
+  const bool dbl = storage == LBM_F64 || a.state64;
   if (dims == 2) {
     if (dbl) hipLaunchKernelGGL((k_contact<double, 2>), g, b, 0, s, a);
     else hipLaunchKernelGGL((k_contact<float, 2>), g, b, 0, s, a);
@@ -572,7 +593,7 @@ hipError_t launch_dem_contact(int dims, int storage, const ParticleArgs& a, hipS
 hipError_t launch_particle_integrate(int dims, int storage, const ParticleArgs& a, hipStream_t s) {
   if (a.n == 0) return hipSuccess;
   const dim3 g((a.n + 255) / 256), b(256);
-  const bool dbl = storage == LBM_F64;
+  const bool dbl = storage == LBM_F64 || a.state64;
   if (dims == 2) {
     if (dbl) hipLaunchKernelGGL((k_integrate<double, 2>), g, b, 0, s, a);
     else hipLaunchKernelGGL((k_integrate<float, 2>), g, b, 0, s, a);
