@@ -40,7 +40,7 @@ hipError_t launch_aa_step(int lattice, int storage, bool odd, const StepArgs& a,
 // A tile is `rows` consecutive rows (from y) of one member, blockDim cells wide
 // from x0.
 struct Tile {
-  uint32_t member, y, z, x0;
+  uint32_t member, c0;  // first cell (natural order) of the tile within the member
 };
 
 struct BatchArgs {

@@ -875,10 +875,8 @@ lbm_status build_tiles(lbm_batch* b) {
     args[m] = member_args(c);
     begin[m] = uint32_t(tiles.size());
     if (!b->active[m]) continue;
-    for (uint64_t z = 0; z < c->n[2]; ++z)
-      for (uint64_t y = 0; y < c->n[1]; y += uint64_t(rows))
-        for (uint64_t x0 = 0; x0 < c->n[0]; x0 += uint64_t(bx))
-          tiles.push_back(Tile{m, uint32_t(y), uint32_t(z), uint32_t(x0)});
+    const uint64_t per = uint64_t(bx) * uint64_t(rows);
+    for (uint64_t c0 = 0; c0 < c->N; c0 += per) tiles.push_back(Tile{m, uint32_t(c0)});
   }
   begin[M] = uint32_t(tiles.size());
   HIP_TRY(hipSetDevice(b->device));

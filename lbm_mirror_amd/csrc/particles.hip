@@ -426,47 +426,64 @@ __device__ inline void soft_shell(const Man<T, D>& m, const T* r_self, const T* 
   cross_rf<T, D>(r_self, F, Tq);
 }
 
+// One 64-lane wavefront per particle: the lanes run the broadphase test for
+// 64 partners at a time and a ballot collects the candidates; lane 0 then
+// takes them in ascending partner order, which is the serial loop's order,
+// so the accumulated force and torque are the serial loop's bit for bit.
 template <class T, int D>
-__global__ void k_contact(ParticleArgs a) {
-  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= a.n) return;
+__global__ __launch_bounds__(64) void k_contact(ParticleArgs a) {
+  const uint32_t s = blockIdx.x;
+  const int lane = int(threadIdx.x);
+  if (s >= a.n) return;  // uniform per block
   T* parts = static_cast<T*>(a.parts);
   T* me = parts + size_t(s) * kPartFields;
   const T steps = T(a.sub_steps == 0 ? 1 : a.sub_steps);
   T Fs[3] = {me[PF_FORCE], me[PF_FORCE + 1], me[PF_FORCE + 2]};
   T Ts = me[PF_TORQUE];
-  for (uint32_t j = 0; j < a.n; ++j) {
-    if (j == s) continue;
-    const T* ot = parts + size_t(j) * kPartFields;
-    // broadphase_collision_check (particle.hpp:247-279)
-    T n2 = T(0);
-    for (int k = 0; k < D; ++k) {
-      const T dd = me[PF_POS + k] - ot[PF_POS + k];
-      n2 = n2 + dd * dd;
+  const T rl = me[PF_RADIUS];
+  for (uint32_t j0 = 0; j0 < a.n; j0 += 64) {
+    const uint32_t jl = j0 + uint32_t(lane);
+    bool cand = false;
+    if (jl < a.n && jl != s) {
+      // broadphase_collision_check (particle.hpp:247-279)
+      const T* ot = parts + size_t(jl) * kPartFields;
+      T n2 = T(0);
+      for (int k = 0; k < D; ++k) {
+        const T dd = me[PF_POS + k] - ot[PF_POS + k];
+        n2 = n2 + dd * dd;
+      }
+      const T rr = ot[PF_RADIUS];
+      const T rab2 = (rl * rl + rr * rr) + (rr * rl) * T(2.0);
+      cand = n2 < rab2;
     }
-    const T rl = me[PF_RADIUS], rr = ot[PF_RADIUS];
-    const T rab2 = (rl * rl + rr * rr) + (rr * rl) * T(2.0);
-    if (!(n2 < rab2)) continue;
-    const bool self_lo = s < j;
-    Man<T, D> m;
-    if (!narrow<T, D>(self_lo ? me : ot, self_lo ? ot : me, m)) continue;
-    // particle_contact_response (contact.hpp:321-364)
-    if (!self_lo)
-      for (int k = 0; k < D; ++k) m.n[k] = -m.n[k];
-    T r_s[3], r_o[3], v_s[3], v_o[3], v_rel[3];
-    for (int k = 0; k < D; ++k) {
-      r_s[k] = m.pt[k] - me[PF_POS + k];
-      r_o[k] = m.pt[k] - ot[PF_POS + k];
+    uint64_t mask = __ballot(cand);
+    if (lane != 0) continue;
+    while (mask) {
+      const uint32_t j = j0 + uint32_t(__ffsll((unsigned long long)mask) - 1);
+      mask &= mask - 1;
+      const T* ot = parts + size_t(j) * kPartFields;
+      const bool self_lo = s < j;
+      Man<T, D> m;
+      if (!narrow<T, D>(self_lo ? me : ot, self_lo ? ot : me, m)) continue;
+      // particle_contact_response (contact.hpp:321-364)
+      if (!self_lo)
+        for (int k = 0; k < D; ++k) m.n[k] = -m.n[k];
+      T r_s[3], r_o[3], v_s[3], v_o[3], v_rel[3];
+      for (int k = 0; k < D; ++k) {
+        r_s[k] = m.pt[k] - me[PF_POS + k];
+        r_o[k] = m.pt[k] - ot[PF_POS + k];
+      }
+      surf_vel<T, D>(me, r_s, steps, v_s);
+      surf_vel<T, D>(ot, r_o, steps, v_o);
+      for (int k = 0; k < D; ++k) v_rel[k] = v_s[k] - v_o[k];
+      const T m_eff = (me[PF_MASS] * ot[PF_MASS]) / (me[PF_MASS] + ot[PF_MASS]);
+      T F[3] = {T(0), T(0), T(0)}, Tq[3] = {T(0), T(0), T(0)};
+      soft_shell<T, D>(m, r_s, v_rel, m_eff, a, F, Tq);
+      for (int k = 0; k < D; ++k) Fs[k] = Fs[k] + F[k];
+      if (D == 2) Ts = Ts + Tq[0];
     }
-    surf_vel<T, D>(me, r_s, steps, v_s);
-    surf_vel<T, D>(ot, r_o, steps, v_o);
-    for (int k = 0; k < D; ++k) v_rel[k] = v_s[k] - v_o[k];
-    const T m_eff = (me[PF_MASS] * ot[PF_MASS]) / (me[PF_MASS] + ot[PF_MASS]);
-    T F[3] = {T(0), T(0), T(0)}, Tq[3] = {T(0), T(0), T(0)};
-    soft_shell<T, D>(m, r_s, v_rel, m_eff, a, F, Tq);
-    for (int k = 0; k < D; ++k) Fs[k] = Fs[k] + F[k];
-    if (D == 2) Ts = Ts + Tq[0];
   }
+  if (lane != 0) return;
   if (a.walls) {  // accumulate_box_wall_contacts (contact.hpp:407-456)
     const int ext[3] = {a.nx, a.ny, a.nz};
     const T r = me[PF_RADIUS];
@@ -495,7 +512,7 @@ __global__ void k_contact(ParticleArgs a) {
       }
     }
   }
-  // only force/torque of this particle are written; no thread reads them
+  // only force/torque of this particle are written; no block reads them
   for (int k = 0; k < D; ++k) me[PF_FORCE + k] = Fs[k];
   if (D == 2) me[PF_TORQUE] = Ts;
 }
@@ -578,7 +595,7 @@ hipError_t launch_psm_couple(int lattice, int storage, const ParticleArgs& a, hi
 
 hipError_t launch_dem_contact(int dims, int storage, const ParticleArgs& a, hipStream_t s) {
   if (a.n == 0) return hipSuccess;
-  const dim3 g((a.n + 255) / 256), b(256);
+  const dim3 g(a.n), b(64);
   const bool dbl = storage == LBM_F64 || a.state64;
   if (dims == 2) {
     if (dbl) hipLaunchKernelGGL((k_contact<double, 2>), g, b, 0, s, a);

@@ -572,8 +572,10 @@ static hipError_t launch_vec(bool odd, const StepArgs& a, hipStream_t s) {
 
 // ---- batched ensembles --------------------------------------------------------
 // Many independent lattices of uneven size, one launch per step: persistent
-// blocks stride over a work list of row tiles ordered member by member; each
-// tile applies exactly the single-lattice row body to its member's fields.
+// blocks stride over a work list of tiles ordered member by member; a tile is
+// rows x blockDim consecutive cells of one member in natural order (x fastest,
+// wrapping across rows), so every lane has a cell whatever the member's width
+// except in the member's last tile; each cell runs the single-lattice body.
 // The step index is read from device memory (t0 + k) so a chain of steps can
 // be captured once in a hipGraph and replayed.
 template <class L, class S, class T, bool WB, bool ODD>
@@ -585,11 +587,12 @@ __global__ __launch_bounds__(256) void k_aa_batch(BatchArgs b) {
     a.t = t;
     a.write_macros = b.write_macros != 0;
     if (b.write_macros == 2) a.u = b.u_prev[tl.member];
-    const int x = int(tl.x0 + threadIdx.x);
+    const uint32_t nx = uint32_t(a.nx), sxy = nx * uint32_t(a.ny);
     for (uint32_t r = 0; r < b.rows; ++r) {
-      const int y = int(tl.y + r);
-      if (y >= a.ny) break;
-      if (x < a.nx) aa_row_cell<L, S, T, WB, ODD>(a, x, y, int(tl.z));
+      const uint32_t c = tl.c0 + r * blockDim.x + threadIdx.x;
+      if (c >= a.N) break;
+      const uint32_t z = c / sxy, rem = c - z * sxy, y = rem / nx;
+      aa_row_cell<L, S, T, WB, ODD>(a, int(rem - y * nx), int(y), int(z));
     }
   }
 }
@@ -603,18 +606,14 @@ __global__ __launch_bounds__(256) void k_batch_residual(BatchArgs b, double* par
     const T* u = static_cast<const T*>(a.u);
     const T* up = static_cast<const T*>(b.u_prev[tl.member]);
     double dsum = 0.0, usum = 0.0;
-    const int x = int(tl.x0 + threadIdx.x);
     for (uint32_t r = 0; r < b.rows; ++r) {
-      const int y = int(tl.y + r);
-      if (y >= a.ny) break;
-      if (x < a.nx) {
-        const uint32_t c = uint32_t(x) + uint32_t(a.nx) * (uint32_t(y) + uint32_t(a.ny) * tl.z);
-        for (int d = 0; d < D; ++d) {
-          const double v = double(u[size_t(d) * a.N + c]);
-          const double w = double(up[size_t(d) * a.N + c]);
-          dsum += (v - w) * (v - w);
-          usum += v * v;
-        }
+      const uint32_t c = tl.c0 + r * blockDim.x + threadIdx.x;
+      if (c >= a.N) break;
+      for (int d = 0; d < D; ++d) {
+        const double v = double(u[size_t(d) * a.N + c]);
+        const double w = double(up[size_t(d) * a.N + c]);
+        dsum += (v - w) * (v - w);
+        usum += v * v;
       }
     }
 #pragma unroll
