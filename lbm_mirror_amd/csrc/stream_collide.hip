@@ -199,6 +199,26 @@ __device__ __forceinline__ void collide_cell(T (&f)[L::Q], uint8_t fl, const Ste
   }
 }
 
+#ifndef LBMX_NT_EVEN
+#define LBMX_NT_EVEN 3
+#endif
+// Phase-0 accesses as nontemporal (bit 0: loads, bit 1: stores), FP32 only.
+template <class S, class T>
+__device__ __forceinline__ T ld_even(const S* p, T w) {
+  if constexpr (sizeof(S) == 4 && sizeof(T) == 4)
+    if (LBMX_NT_EVEN & 1) return __builtin_nontemporal_load(p);
+  return Store<S, T>::ld(p, w);
+}
+template <class S, class T>
+__device__ __forceinline__ void st_even(S* p, T v, T w) {
+  if constexpr (sizeof(S) == 4 && sizeof(T) == 4)
+    if (LBMX_NT_EVEN & 2) {
+      __builtin_nontemporal_store(v, p);
+      return;
+    }
+  Store<S, T>::st(p, v, w);
+}
+
 template <class L, class S, class T, bool WB, bool ODD>
 __global__ __launch_bounds__(256) void k_aa_step(StepArgs a) {
   constexpr int Q = L::Q;
@@ -227,7 +247,7 @@ __global__ __launch_bounds__(256) void k_aa_step(StepArgs a) {
   T f[Q];
   if (!ODD) {
 #pragma unroll
-    for (int i = 0; i < Q; ++i) f[i] = Store<S, T>::ld(fb + size_t(i) * Np + cell, T(L::w[i]));
+    for (int i = 0; i < Q; ++i) f[i] = ld_even<S, T>(fb + size_t(i) * Np + cell, T(L::w[i]));
   } else {
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
@@ -246,7 +266,7 @@ __global__ __launch_bounds__(256) void k_aa_step(StepArgs a) {
   if (!ODD) {
 #pragma unroll
     for (int i = 0; i < Q; ++i)
-      Store<S, T>::st(fb + size_t(L::opp[i]) * Np + cell, f[i], T(L::w[i]));
+      st_even<S, T>(fb + size_t(L::opp[i]) * Np + cell, f[i], T(L::w[i]));
   } else {
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
