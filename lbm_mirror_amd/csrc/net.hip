@@ -176,46 +176,68 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
 }
 
 struct WgradArgs {
-  int B, H, W, cin, cout;  // M' = 9*cin, N' = cout, K' = pixels
+  int B, H, W, cin, cout;  // M' = 9*cin (+ the bias row), N' = cout, K' = pixels
   Act in, dpre;
-  float* partial;  // [splits][9*cin][cout]
+  float* partial;  // [splits][9*cin + 8][cout]
   int chunk;       // pixels per split, multiple of BK
 };
 
-// Weight gradient: per split of the pixel range, one partial 128x32 tile.
+// Two 8-element chunks (pixels k and k+1 of the same 8 rows) into LDS rows
+// base[e*stride], as one 32-bit write per row.
+__device__ inline void scatter8x2(uint16_t* base, int stride, uint4 v0, uint4 v1) {
+  const uint32_t w0[4] = {v0.x, v0.y, v0.z, v0.w}, w1[4] = {v1.x, v1.y, v1.z, v1.w};
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    *reinterpret_cast<uint32_t*>(base + (2 * t) * stride) = (w1[t] << 16) | (w0[t] & 0xffffu);
+    *reinterpret_cast<uint32_t*>(base + (2 * t + 1) * stride) = (w1[t] & 0xffff0000u) | (w0[t] >> 16);
+  }
+}
+
+// Weight gradient: per split of the pixel range, one partial 128x32 tile of
+// dW[(tap,ci)][co] = sum_p in[p+off(tap)][ci] * dpre[p][co].  Row 9*cin of the
+// im2col operand is 1 on every pixel, so the same GEMM yields the bias
+// gradient sum_p dpre[p][co] in that row (rows 9*cin+1 .. +7 are zero).
 __global__ __launch_bounds__(256) void k_wgrad(WgradArgs a) {
   __shared__ __align__(16) uint16_t As[BM * LK];
   __shared__ __align__(16) uint16_t Bs[BN * LK];
-  const int M = a.B * a.H * a.W, Mw = 9 * a.cin, N = a.cout, HW = a.H * a.W;
+  const int M = a.B * a.H * a.W, Mw = 9 * a.cin, Mx = Mw + 8, N = a.cout, HW = a.H * a.W;
   const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN, s = blockIdx.z;
   const int p0 = s * a.chunk, p1 = min(p0 + a.chunk, M);
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int g = tid & 15;
   const int m8 = m0 + g * 8;
-  const bool mv = m8 < Mw;
+  const bool mv = m8 < Mw, ones = m8 == Mw;
   const int tap = mv ? m8 / a.cin : 0;
   const int ci = mv ? m8 - tap * a.cin : 0;
-  uint4 ra[2], rbv = make_uint4(0u, 0u, 0u, 0u);
+  const int kq = 2 * (tid >> 4);              // A: pixel pair (kq, kq+1) of the K step
+  const int kb = 2 * ((tid & 63) >> 2), nc = (tid & 3) * 8;  // B: pixel pair, channel chunk (tid < 64)
+  uint4 ra[2], rb[2];
   auto gload = [&](int pb) {
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
-      const int q = pb + (tid >> 4) + 16 * j;
+      const int q = pb + kq + j;
       ra[j] = make_uint4(0u, 0u, 0u, 0u);
-      if (mv && q < p1) {
-        const int b = q / HW, r = q - b * HW, y = r / a.W, x = r - y * a.W;
-        ra[j] = load_tap(a.in, a.H, a.W, b, y, x, tap, ci);
+      if (q < p1) {
+        if (mv) {
+          const int b = q / HW, r = q - b * HW, y = r / a.W, x = r - y * a.W;
+          ra[j] = load_tap(a.in, a.H, a.W, b, y, x, tap, ci);
+        } else if (ones) {
+          ra[j].x = 0x3f80u;  // bf16 1.0 in row 9*cin
+        }
       }
     }
-    if (tid < 128) {
-      const int q = pb + (tid >> 2), n = n0 + (tid & 3) * 8;
-      rbv = (q < p1 && n < N) ? *reinterpret_cast<const uint4*>(a.dpre.p + size_t(q) * a.dpre.ld + a.dpre.off + n)
-                              : make_uint4(0u, 0u, 0u, 0u);
+    if (tid < 64) {
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int q = pb + kb + j, n = n0 + nc;
+        rb[j] = (q < p1 && n < N) ? *reinterpret_cast<const uint4*>(a.dpre.p + size_t(q) * a.dpre.ld + a.dpre.off + n)
+                                  : make_uint4(0u, 0u, 0u, 0u);
+      }
     }
   };
   auto sstore = [&]() {
-#pragma unroll
-    for (int j = 0; j < 2; ++j) scatter8(&As[(g * 8) * LK + (tid >> 4) + 16 * j], LK, ra[j]);
-    if (tid < 128) scatter8(&Bs[((tid & 3) * 8) * LK + (tid >> 2)], LK, rbv);
+    scatter8x2(&As[(g * 8) * LK + kq], LK, ra[0], ra[1]);
+    if (tid < 64) scatter8x2(&Bs[nc * LK + kb], LK, rb[0], rb[1]);
   };
   f32x4 acc[2][2];
 #pragma unroll
@@ -237,7 +259,7 @@ __global__ __launch_bounds__(256) void k_wgrad(WgradArgs a) {
       __syncthreads();
     }
   }
-  float* out = a.partial + size_t(s) * Mw * N;
+  float* out = a.partial + size_t(s) * Mx * N;
 #pragma unroll
   for (int r = 0; r < 2; ++r)
 #pragma unroll
@@ -246,40 +268,20 @@ __global__ __launch_bounds__(256) void k_wgrad(WgradArgs a) {
       for (int j = 0; j < 4; ++j) {
         const int m = m0 + wave * 32 + r * 16 + (lane >> 4) * 4 + j;
         const int n = n0 + c * 16 + (lane & 15);
-        if (m < Mw && n < N) out[size_t(m) * N + n] = acc[r][c][j];
+        if (m <= Mw && n < N) out[size_t(m) * N + n] = acc[r][c][j];
       }
 }
 
-// grad[m*cr + n] = sum_s partial[s][m][n], n < cr (the unpadded Cout).
+// grad[m*cr + n] = sum_s partial[s][m][n] for m <= 9*cin, n < cr (the
+// unpadded Cout); row 9*cin is the bias, which follows the weights in the
+// flat parameter layout.
 __global__ void k_wreduce(const float* partial, int splits, int Mw, int cpad, int cr, float* grad) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= Mw * cr) return;
+  if (i >= (Mw + 1) * cr) return;
   const int m = i / cr, n = i - m * cr;
   float s = 0.f;
-  for (int k = 0; k < splits; ++k) s += partial[(size_t(k) * Mw + m) * cpad + n];
+  for (int k = 0; k < splits; ++k) s += partial[(size_t(k) * (Mw + 8) + m) * cpad + n];
   grad[i] = s;
-}
-
-// Bias gradient, stage 1: partial[s][c] = sum over the split's pixels.
-__global__ void k_colsum(Act d, int M, int C, int chunk, float* partial) {
-  __shared__ float red[4][64];
-  const int c = blockIdx.x * 64 + threadIdx.x, s = blockIdx.y;
-  const int p0 = s * chunk, p1 = min(p0 + chunk, M);
-  float v = 0.f;
-  if (c < C)
-    for (int q = p0 + int(threadIdx.y); q < p1; q += 4) v += bf2f(d.p[size_t(q) * d.ld + d.off + c]);
-  red[threadIdx.y][threadIdx.x] = v;
-  __syncthreads();
-  if (threadIdx.y == 0 && c < C)
-    partial[size_t(s) * C + c] = ((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) + red[3][threadIdx.x];
-}
-
-__global__ void k_colreduce(const float* partial, int splits, int C, int cr, float* grad) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= cr) return;
-  float s = 0.f;
-  for (int k = 0; k < splits; ++k) s += partial[size_t(k) * C + c];
-  grad[c] = s;
 }
 
 // 2x2 average pool of C channels: fine (H x W) -> coarse (H/2 x W/2).
@@ -503,8 +505,6 @@ struct lbm_net {
   uint8_t* msk = nullptr;
   float* partial = nullptr;
   size_t partial_n = 0;
-  float* bpartial = nullptr;
-  size_t bpartial_n = 0;
   double* lpartial = nullptr;
   double* dloss = nullptr;
   float* dlosses = nullptr;
@@ -571,7 +571,7 @@ Act act(uint16_t* p, int ld, int off = 0) { return Act{p, ld, off}; }
 // Splits of the pixel range for a weight-gradient launch of layer L.
 void wgrad_shape(const lbm_net* n, const Layer& L, int B, int& splits, int& chunk) {
   const int M = B * H_(n, L.level) * W_(n, L.level);
-  const int tiles = ((9 * L.cin + BM - 1) / BM) * ((L.cpad + BN - 1) / BN);
+  const int tiles = ((9 * L.cin + 8 + BM - 1) / BM) * ((L.cpad + BN - 1) / BN);
   splits = std::max(1, std::min(1024 / std::max(1, tiles), (M + 255) / 256));
   chunk = ((M + splits - 1) / splits + BK - 1) / BK * BK;
   splits = (M + chunk - 1) / chunk;
@@ -610,20 +610,12 @@ lbm_status conv_bwd(lbm_net* n, int li, int B, Act in, Act dpre, Act dx, bool ma
   int splits, chunk;
   wgrad_shape(n, L, B, splits, chunk);
   WgradArgs wa{B, H, W, L.cin, L.cpad, in, dpre, n->partial, chunk};
-  dim3 wg((9 * L.cin + BM - 1) / BM, (L.cpad + BN - 1) / BN, splits);
+  dim3 wg((9 * L.cin + 8 + BM - 1) / BM, (L.cpad + BN - 1) / BN, splits);
   hipLaunchKernelGGL(k_wgrad, wg, dim3(256), 0, n->stream, wa);
   NET_TRY(hipGetLastError());
   const int Mw = 9 * L.cin;
-  hipLaunchKernelGGL(k_wreduce, dim3((Mw * L.cr + 255) / 256), dim3(256), 0, n->stream, n->partial, splits, Mw, L.cpad,
-                     L.cr, n->g + L.w_off);
-  NET_TRY(hipGetLastError());
-  const int bs = std::max(1, std::min(256, (M + 1023) / 1024));
-  const int bchunk = (M + bs - 1) / bs;
-  hipLaunchKernelGGL(k_colsum, dim3((L.cpad + 63) / 64, bs), dim3(64, 4), 0, n->stream, dpre, M, L.cpad, bchunk,
-                     n->bpartial);
-  NET_TRY(hipGetLastError());
-  hipLaunchKernelGGL(k_colreduce, dim3((L.cr + 63) / 64), dim3(64), 0, n->stream, n->bpartial, bs, L.cpad, L.cr,
-                     n->g + L.b_off);
+  hipLaunchKernelGGL(k_wreduce, dim3(((Mw + 1) * L.cr + 255) / 256), dim3(256), 0, n->stream, n->partial, splits, Mw,
+                     L.cpad, L.cr, n->g + L.w_off);
   NET_TRY(hipGetLastError());
   if (dx.p) {
     ConvArgs a{};
@@ -815,7 +807,7 @@ void free_all(lbm_net* n) {
     (void)hipFree(L.wt);
   }
   void* ps[] = {n->w,      n->g,       n->m1,      n->m2,       n->x,       n->bott,      n->gbott, n->gy,
-                n->y,      n->t,       n->msk,     n->partial,  n->bpartial, n->lpartial, n->dloss,  n->dlosses,
+                n->y,      n->t,       n->msk,     n->partial,  n->lpartial, n->dloss,  n->dlosses,
                 n->didx,   n->xs,      n->ts,      n->ms};
   for (void* p : ps) (void)hipFree(p);
   for (int l = 0; l < 4; ++l) {
@@ -873,17 +865,14 @@ lbm_status create_impl(lbm_net* n) {
   NS(dalloc(&n->y, level_pixels(n, 0) * 3));
   NS(dalloc(&n->t, level_pixels(n, 0) * 3));
   NS(dalloc(&n->msk, level_pixels(n, 0)));
-  size_t pmax = 0, bmax = 0;
+  size_t pmax = 0;
   for (const Layer& L : n->layers) {
     int splits, chunk;
     wgrad_shape(n, L, c.batch, splits, chunk);
-    pmax = std::max(pmax, size_t(splits) * 9 * L.cin * L.cpad);
-    bmax = std::max(bmax, size_t(256) * L.cpad);
+    pmax = std::max(pmax, size_t(splits) * (9 * L.cin + 8) * L.cpad);
   }
   n->partial_n = pmax;
-  n->bpartial_n = bmax;
   NS(dalloc(&n->partial, pmax));
-  NS(dalloc(&n->bpartial, bmax));
   NS(dalloc(&n->lpartial, size_t(blocks_for(level_pixels(n, 0)))));
   NS(dalloc(&n->dloss, 1));
   const size_t HW = size_t(c.height) * c.width;

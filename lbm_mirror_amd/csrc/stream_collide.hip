@@ -319,7 +319,7 @@ __device__ __forceinline__ void aa_row_cell(const StepArgs& a, int x, int y, int
   T f[Q];
   if (!ODD) {
 #pragma unroll
-    for (int i = 0; i < Q; ++i) f[i] = Store<S, T>::ld(rowp(i, 1, 1) + X[1], T(L::w[i]));
+    for (int i = 0; i < Q; ++i) f[i] = ld_even<S, T>(rowp(i, 1, 1) + X[1], T(L::w[i]));
   } else if (inner) {
 #pragma unroll
     for (int i = 0; i < Q; ++i)
@@ -337,7 +337,7 @@ __device__ __forceinline__ void aa_row_cell(const StepArgs& a, int x, int y, int
 
   if (!ODD) {
 #pragma unroll
-    for (int i = 0; i < Q; ++i) Store<S, T>::st(rowp(L::opp[i], 1, 1) + X[1], f[i], T(L::w[i]));
+    for (int i = 0; i < Q; ++i) st_even<S, T>(rowp(L::opp[i], 1, 1) + X[1], f[i], T(L::w[i]));
   } else if (inner) {
 #pragma unroll
     for (int i = 0; i < Q; ++i)
