@@ -121,14 +121,18 @@ def bench_c3(args, storage):
     discs = seeded_discs(200, (nx, ny), 8.0, 16.0, seed=20260103)
     steps = args.steps if args.steps != 1000 else 2000
 
+    hlbm = args.coupling == "hlbm"
+    fluid = abi.COLL_HLBM if hlbm else abi.COLL_PSM
+    edge = abi.COLL_HLBM if hlbm else abi.COLL_BGK
+
     def run(st, s64=False):
         lat = Lattice(abi.D2Q9, st, (nx, ny))
         lat.set_flags(channel_flags_2d(nx, ny))
         ops = {1: abi.op_desc(boundary=abi.BC_BOUNCE_BACK),
-               2: abi.op_desc(collision=abi.COLL_PSM, tau=0.7),
-               3: abi.op_desc(boundary=abi.BC_ZOUHE_PRESSURE_X, collision=abi.COLL_BGK, east=1, tau=0.7,
+               2: abi.op_desc(collision=fluid, tau=0.7),
+               3: abi.op_desc(boundary=abi.BC_ZOUHE_PRESSURE_X, collision=edge, east=1, tau=0.7,
                               ramp=(1.0, 0.001 / 64, 1.001, 64)),
-               4: abi.op_desc(boundary=abi.BC_ZOUHE_PRESSURE_X, collision=abi.COLL_BGK, east=0,
+               4: abi.op_desc(boundary=abi.BC_ZOUHE_PRESSURE_X, collision=edge, east=0,
                               tau=0.7, rho=1.0)}
         for fl, op in ops.items():
             lat.set_operator(fl, op)
@@ -142,7 +146,8 @@ def bench_c3(args, storage):
             p.mass, p.inertia = spheroid_mass(2, abi.F64 if (st == abi.F64 or s64) else abi.F32, r, 1.5)
             parts.append(p)
         prm = abi.particle_params(k_n=1e3, zeta_n=restitution_to_zeta(0.9), walls=True, sub_steps=16,
-                                  body_acceleration=(0.0, -1e-6), eps=1.5, max_radius=16.0, state_fp64=s64)
+                                  body_acceleration=(0.0, -1e-6), eps=1.5, max_radius=16.0, state_fp64=s64,
+                                  coupling=abi.COUPLING_HLBM if hlbm else abi.COUPLING_PSM)
         lat.particles_set(parts, prm)
         lat.step(args.warmup, t0=0)
         lat.sync()
@@ -162,14 +167,14 @@ def bench_c3(args, storage):
     err_u64 = float(np.linalg.norm(u_s64 - u64) / np.linalg.norm(u64))
     err_x64 = float(np.max(np.linalg.norm(pos_s64 - pos64, axis=1)))
     sb = {abi.F32: 4, abi.F16S: 2, abi.BF16S: 2}[storage]
-    return {"metric": "MLUPS (D2Q9 PSM particle-laden channel, full step incl. particles)",
+    return {"metric": "MLUPS (D2Q9 %s particle-laden channel, full step incl. particles)" % ("HLBM" if hlbm else "PSM"),
             "value": round(nx * ny * steps / (ms * 1e3), 1), "unit": "MLUPS", "n_gpus": 1,
             "steps": steps, "warmup": args.warmup, "ms_per_step": round(ms / steps, 5),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": {abi.F32: "f32", abi.F16S: "f16-storage/f32", abi.BF16S: "bf16-storage/f32"}[storage],
             "data": "synthetic",
-            "config": {"workload": "D2Q9 PSM 2048x512, 200 discs r~U[8,16], rho_p 1.5, g 1e-6, "
-                                   "DEM k 1e3 e 0.9, tau 0.7",
+            "config": {"workload": "D2Q9 %s 2048x512, 200 discs r~U[8,16], rho_p 1.5, g 1e-6, "
+                                   "DEM k 1e3 e 0.9, tau 0.7" % ("HLBM" if hlbm else "PSM"),
                        "error_vs_fp64": {"velocity_rel_l2": err_u, "trajectory_max_cells": err_x,
                                          "max_displacement_cells": disp},
                        "with_fp64_particle_state": {"velocity_rel_l2": err_u64, "trajectory_max_cells": err_x64,
@@ -368,6 +373,8 @@ def main():
     ap.add_argument("--train-max-steps", type=int, default=100000)
     ap.add_argument("--test-max-steps", type=int, default=400000)
     ap.add_argument("--members", type=int, default=256)
+    ap.add_argument("--coupling", default="psm", choices=["psm", "hlbm"],
+                    help="particle coupling of --config c3")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs one lattice on one GPU; multi-GPU "

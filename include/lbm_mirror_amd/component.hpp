@@ -34,6 +34,7 @@ namespace cmpt {  // component kinds, reference names
 struct BGK {};
 struct BGKGuo {};
 struct Psm {};
+struct Hlbm {};
 struct BounceBack {};
 template <uint64_t I> struct AntiBounceBack {};
 template <bool East> struct ZouHePressureX {};
@@ -81,6 +82,20 @@ class component<T, Desc, cmpt::BGKGuo> {
   double tau_;
   std::array<double, 3> g_;
   bool cell_;
+};
+
+// cmpt::Hlbm (hlbm.hpp:62-105): BGK towards eq(rho, u*B + N(1-B)/D); the
+// particle side is selected with lbm_particle_params::coupling = LBM_COUPLING_HLBM.
+template <typename T, typename Desc>
+class component<T, Desc, cmpt::Hlbm> {
+ public:
+  explicit component(double relaxation) : tau_(relaxation) {}
+  void fill(lbm_op_desc& d) const {
+    d.collision = LBM_COLL_HLBM;
+    d.tau = tau_;
+  }
+ private:
+  double tau_;
 };
 
 template <typename T, typename Desc>

@@ -76,8 +76,14 @@ typedef enum lbm_collision_kind {
   LBM_COLL_NONE = 0,
   LBM_COLL_BGK = 1,      /* cmpt::BGK     collision.hpp:76-96            */
   LBM_COLL_BGK_GUO = 2,  /* cmpt::BGKGuo  collision.hpp:135-204          */
-  LBM_COLL_PSM = 3       /* cmpt::Psm     psm.hpp:58-89                  */
+  LBM_COLL_PSM = 3,      /* cmpt::Psm     psm.hpp:58-89                  */
+  LBM_COLL_HLBM = 4      /* cmpt::Hlbm    hlbm.hpp:62-105                */
 } lbm_collision_kind;
+
+typedef enum lbm_particle_coupling {
+  LBM_COUPLING_PSM = 0,  /* PsmReset + PsmOneParticleGather (psm.hpp:20-32,101-445)        */
+  LBM_COUPLING_HLBM = 1  /* HlbmReset + HlbmOneParticleGather/Stamp (hlbm.hpp:28-49,111-325) */
+} lbm_particle_coupling;
 
 /* lbm_op_desc.flags */
 #define LBM_OPF_GUO_CELL_FORCE  0x1u  /* BGKGuo also reads the per-cell force field (collision.hpp:260-263) */
@@ -135,7 +141,7 @@ typedef struct lbm_particle_params {
                                        compute type, as the reference (default);
                                        1: FP64 particle state and arithmetic whatever
                                        the storage (no reference counterpart)       */
-  int32_t reserved;
+  int32_t coupling;                 /* lbm_particle_coupling (HLBM needs scatter porosity) */
 } lbm_particle_params;
 
 typedef struct lbm_ctx lbm_ctx;

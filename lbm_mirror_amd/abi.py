@@ -34,7 +34,8 @@ BC_NONE, BC_BOUNCE_BACK, BC_ANTI_BOUNCE_BACK = 0, 1, 2
 BC_ZOUHE_PRESSURE_X, BC_ZOUHE_VELOCITY_X, BC_EQUILIBRIUM = 3, 4, 5
 
 # lbm_collision_kind
-COLL_NONE, COLL_BGK, COLL_BGK_GUO, COLL_PSM = 0, 1, 2, 3
+COLL_NONE, COLL_BGK, COLL_BGK_GUO, COLL_PSM, COLL_HLBM = 0, 1, 2, 3, 4
+COUPLING_PSM, COUPLING_HLBM = 0, 1
 
 OPF_GUO_CELL_FORCE = 0x1
 OPF_PSM_STAMP_DISC = 0x2
@@ -93,7 +94,7 @@ class ParticleParams(C.Structure):
         ("eps", C.c_double),
         ("max_radius", C.c_double),
         ("state_fp64", C.c_int32),
-        ("reserved", C.c_int32),
+        ("coupling", C.c_int32),
     ]
 
 
@@ -120,7 +121,7 @@ def op_desc(boundary=BC_NONE, collision=COLL_NONE, east=0, flags=0, tau=1.0,
 def particle_params(k_n=1.0, zeta_n=0.3, gamma_t=0.0, mu=0.0, walls=False,
                     porosity_mode=POROSITY_SCATTER, psm_torque=False,
                     cell_capacity=4, sub_steps=1, body_acceleration=(0.0, 0.0, 0.0),
-                    eps=1.5, max_radius=1.0, state_fp64=False):
+                    eps=1.5, max_radius=1.0, state_fp64=False, coupling=COUPLING_PSM):
     p = ParticleParams()
     p.k_n, p.zeta_n, p.gamma_t, p.mu = k_n, zeta_n, gamma_t, mu
     p.walls, p.porosity_mode, p.psm_torque = int(walls), porosity_mode, int(psm_torque)
@@ -129,6 +130,7 @@ def particle_params(k_n=1.0, zeta_n=0.3, gamma_t=0.0, mu=0.0, walls=False,
         p.body_acceleration[i] = body_acceleration[i] if i < len(body_acceleration) else 0.0
     p.eps, p.max_radius = eps, max_radius
     p.state_fp64 = int(state_fp64)
+    p.coupling = coupling
     return p
 
 

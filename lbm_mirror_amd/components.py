@@ -39,6 +39,14 @@ class Psm:
     cell_force: bool = False
 
 
+@dataclass
+class Hlbm:
+    """cmpt::Hlbm (hlbm.hpp:62-105): BGK towards eq(rho, u*B + N(1-B)/D).  The
+    particle side (HlbmReset, HlbmOneParticleGather/Stamp) is selected with
+    ``abi.particle_params(coupling=abi.COUPLING_HLBM)``."""
+    tau: float
+
+
 # ---- boundaries -------------------------------------------------------------
 @dataclass
 class BounceBack:
@@ -96,6 +104,8 @@ def combine(boundary=None, collision=None):
         kw.update(collision=abi.COLL_BGK_GUO, tau=collision.tau, vec=tuple(collision.global_force))
         if collision.cell_force:
             flags |= abi.OPF_GUO_CELL_FORCE
+    elif isinstance(collision, Hlbm):
+        kw.update(collision=abi.COLL_HLBM, tau=collision.tau)
     elif isinstance(collision, Psm):
         kw.update(collision=abi.COLL_PSM, tau=collision.tau)
         if collision.stamp_disc is not None:

@@ -21,6 +21,8 @@ struct StepArgs {
   void* force;           // D*N
   void* por;             // N
   const void* bcvel;     // D*N
+  const void* pn;        // D*N HLBM particle velocity sums N
+  const void* pd;        // N   HLBM particle weights D
   int32_t nx, ny, nz;
   uint32_t N, Np;
   uint32_t periodic;
@@ -104,6 +106,9 @@ struct ParticleArgs {
   uint64_t grid[3];
   uint32_t cap;
   int32_t state64;  // particle records and arithmetic in FP64 whatever the storage
+  int32_t hlbm;     // HLBM coupling: raw momentum as cell force, torque, stamp of N and D
+  void* pn;         // D*N HLBM N sums (fluid field type)
+  void* pd;         // N   HLBM D sums
 };
 
 // Per-particle record layout (T values)
@@ -114,6 +119,8 @@ enum PartField {
 };
 
 hipError_t launch_psm_couple(int lattice, int storage, const ParticleArgs& a, hipStream_t s);
+// HLBM stamp after the move: N += u_p (1 - B), D += 1 - B over each post-move AABB
+hipError_t launch_hlbm_stamp(int dims, int storage, const ParticleArgs& a, hipStream_t s);
 hipError_t launch_dem_contact(int dims, int storage, const ParticleArgs& a, hipStream_t s);
 hipError_t launch_particle_integrate(int dims, int storage, const ParticleArgs& a, hipStream_t s);
 

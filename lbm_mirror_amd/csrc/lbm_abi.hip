@@ -78,6 +78,8 @@ struct lbm_ctx {
   void* force = nullptr;
   void* por = nullptr;
   void* bcvel = nullptr;
+  void* pn = nullptr;  // HLBM N sums (D*N)
+  void* pd = nullptr;  // HLBM D sums (N)
   uint32_t periodic = 0;
   int phase = 0;
   bool flags_set = false;
@@ -123,6 +125,18 @@ lbm_status ensure_field(lbm_ctx* c, void** field, size_t count, double fill) {
     HIP_TRY(hipFree(tmp));
   }
   return LBM_OK;
+}
+
+// Porosity (1), cell force (0) and the HLBM sums N, D (0): what the HLBM
+// collision and the HLBM particle coupling read and write.
+lbm_status ensure_hlbm_fields(lbm_ctx* c) {
+  lbm_status s = ensure_field(c, &c->por, c->N, 1.0);
+  if (s) return s;
+  s = ensure_field(c, &c->force, size_t(c->D) * c->N, 0.0);
+  if (s) return s;
+  s = ensure_field(c, &c->pn, size_t(c->D) * c->N, 0.0);
+  if (s) return s;
+  return ensure_field(c, &c->pd, c->N, 0.0);
 }
 
 Geometry geom(const lbm_ctx* c) {
@@ -200,6 +214,10 @@ ParticleArgs particle_args(lbm_ctx* c) {
   a.bin = c->bin;
   for (int d = 0; d < 3; ++d) a.grid[d] = c->grid[d];
   a.cap = uint32_t(c->pp.cell_capacity);
+  a.hlbm = c->pp.coupling == LBM_COUPLING_HLBM ? 1 : 0;
+  a.pn = c->pn;
+  a.pd = c->pd;
+  if (a.hlbm) a.eps = to_T(pst, 1.5);  // HlbmOneParticleGather's fixed ramp width (hlbm.hpp:173)
   return a;
 }
 
@@ -221,6 +239,7 @@ lbm_status particle_pass(lbm_ctx* c, uint64_t t) {
   HIP_TRY(launch_psm_couple(c->lattice, c->storage, a, c->stream));
   HIP_TRY(launch_dem_contact(c->D, c->storage, a, c->stream));
   HIP_TRY(launch_particle_integrate(c->D, c->storage, a, c->stream));
+  if (a.hlbm) HIP_TRY(launch_hlbm_stamp(c->D, c->storage, a, c->stream));
   return LBM_OK;
 }
 
@@ -234,6 +253,8 @@ lbm_status fluid_pass(lbm_ctx* c, uint64_t t, bool write_macros) {
   a.force = c->force;
   a.por = c->por;
   a.bcvel = c->bcvel;
+  a.pn = c->pn;
+  a.pd = c->pd;
   a.nx = int32_t(c->n[0]);
   a.ny = int32_t(c->n[1]);
   a.nz = int32_t(c->n[2]);
@@ -321,7 +342,7 @@ lbm_status lbm_ctx_destroy(lbm_ctx* c) {
   if (!c) return LBM_OK;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  void* ptrs[] = {c->f, c->flags, c->ops, c->rho, c->u, c->force, c->por, c->bcvel,
+  void* ptrs[] = {c->f, c->flags, c->ops, c->rho, c->u, c->force, c->por, c->bcvel, c->pn, c->pd,
                   c->parts, c->f64, c->t64, c->partial, c->cl_count, c->cl_slot,
                   c->cl_overflow, c->owner};
   for (void* p : ptrs)
@@ -367,7 +388,7 @@ lbm_status lbm_set_operator(lbm_ctx* c, uint8_t flag, const lbm_op_desc* op) {
   if (flag >= kMaxOps) return fail(LBM_ERR_NOT_SUPPORTED, "operators on flags >= %d", kMaxOps);
   if (op->boundary < LBM_BC_NONE || op->boundary > LBM_BC_EQUILIBRIUM)
     return fail(LBM_ERR_INVALID_ARG, "unknown boundary kind %d", op->boundary);
-  if (op->collision < LBM_COLL_NONE || op->collision > LBM_COLL_PSM)
+  if (op->collision < LBM_COLL_NONE || op->collision > LBM_COLL_HLBM)
     return fail(LBM_ERR_INVALID_ARG, "unknown collision kind %d", op->collision);
   if (op->boundary == LBM_BC_ZOUHE_PRESSURE_X && c->lattice == LBM_D3Q19)
     return fail(LBM_ERR_NOT_SUPPORTED, "the reference has Zou/He pressure for D2Q9 and D3Q27 only");
@@ -394,6 +415,10 @@ lbm_status lbm_set_operator(lbm_ctx* c, uint8_t flag, const lbm_op_desc* op) {
   o.force_mask = 0;
   for (int d = 0; d < c->D; ++d)
     if ((o.flags & LBM_OPF_GUO_CELL_FORCE) || o.vec[d] != 0.0) o.force_mask |= 1u << d;
+  if (o.coll == LBM_COLL_HLBM) {
+    lbm_status s = ensure_hlbm_fields(c);
+    if (s) return s;
+  }
   if (o.coll == LBM_COLL_PSM) {
     lbm_status s = ensure_field(c, &c->por, c->N, 1.0);
     if (s) return s;
@@ -634,6 +659,10 @@ lbm_status lbm_particles_set(lbm_ctx* c, const lbm_particle* p, uint32_t n,
     return fail(LBM_ERR_INVALID_ARG, "porosity mode %d", prm->porosity_mode);
   if (prm->cell_capacity <= 0 && prm->porosity_mode == LBM_POROSITY_PRODUCT)
     return fail(LBM_ERR_INVALID_ARG, "cell capacity must be positive");
+  if (prm->coupling != LBM_COUPLING_PSM && prm->coupling != LBM_COUPLING_HLBM)
+    return fail(LBM_ERR_INVALID_ARG, "coupling %d", prm->coupling);
+  if (prm->coupling == LBM_COUPLING_HLBM && prm->porosity_mode != LBM_POROSITY_SCATTER)
+    return fail(LBM_ERR_NOT_SUPPORTED, "HLBM assigns porosity per particle (scatter mode only)");
   HIP_TRY(hipSetDevice(c->device));
   HIP_TRY(hipStreamSynchronize(c->stream));
   const int st = particle_storage(c->storage, *prm);
@@ -671,6 +700,10 @@ lbm_status lbm_particles_set(lbm_ctx* c, const lbm_particle* p, uint32_t n,
     if (s) return s;
     s = ensure_field(c, &c->force, size_t(c->D) * c->N, 0.0);
     if (s) return s;
+    if (prm->coupling == LBM_COUPLING_HLBM) {
+      s = ensure_hlbm_fields(c);
+      if (s) return s;
+    }
     if (!c->owner) HIP_TRY(alloc_zero((void**)&c->owner, size_t(c->N) * 2 * sizeof(uint32_t), c->stream));
   }
   // cell list (make_cell_list, particle/cell_list.hpp:185-206) in T
@@ -847,6 +880,8 @@ StepArgs member_args(const lbm_ctx* c) {
   a.force = c->force;
   a.por = c->por;
   a.bcvel = c->bcvel;
+  a.pn = c->pn;
+  a.pd = c->pd;
   a.nx = int32_t(c->n[0]);
   a.ny = int32_t(c->n[1]);
   a.nz = int32_t(c->n[2]);

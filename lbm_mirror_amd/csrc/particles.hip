@@ -132,6 +132,43 @@ __global__ void k_clear(float* por_f, double* por_d, uint32_t* owner, uint32_t N
   owner[2 * c + 1] = 0;
 }
 
+// HlbmReset (hlbm.hpp:28-49): N = 0, D = 0 (porosity in k_clear)
+template <class F>
+__global__ void k_hlbm_reset(F* pn, F* pd, uint32_t N, int dims) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  pd[c] = F(0);
+  for (int d = 0; d < dims; ++d) pn[size_t(d) * N + c] = F(0);
+}
+
+// HlbmOneParticleStamp (hlbm.hpp:264-308): over the post-move AABB,
+// D += 1 - B and N += u_p (1 - B) with B the porosity of this step's gather
+// and u_p = (X - X_old) / ts after the move.  Cells where two discs' solid
+// fractions overlap take both contributions by atomic addition.
+template <class T, class F, int D>
+__global__ __launch_bounds__(256) void k_hlbm_stamp(ParticleArgs a) {
+  const uint32_t p = blockIdx.x / a.blocks_per_particle;
+  const uint32_t chunk = blockIdx.x % a.blocks_per_particle;
+  if (p >= a.n) return;
+  const T* P = static_cast<const T*>(a.parts) + size_t(p) * kPartFields;
+  const Box<T, D> b = aabb_of<T, D>(P, a);
+  const T ts = T(1.0) / T(a.sub_steps == 0 ? uint64_t(1) : a.sub_steps);
+  T vp[3] = {T(0), T(0), T(0)};
+  for (int d = 0; d < D; ++d) vp[d] = (P[PF_POS + d] - P[PF_POS_OLD + d]) / ts;
+  const F* por = static_cast<const F*>(a.por);
+  F* pn = static_cast<F*>(a.pn);
+  F* pd = static_cast<F*>(a.pd);
+  for (uint32_t k = chunk * blockDim.x + threadIdx.x; k < b.count; k += a.blocks_per_particle * blockDim.x) {
+    int x[3];
+    box_cell<T, D>(b, k, x);
+    const uint32_t c = uint32_t(x[0]) + uint32_t(a.nx) * (uint32_t(x[1]) + uint32_t(a.ny) * uint32_t(x[2]));
+    const F flip = F(1.0) - por[c];
+    if (flip == F(0)) continue;
+    atomicAdd(pd + c, flip);
+    for (int d = 0; d < D; ++d) atomicAdd(pn + size_t(d) * a.N + c, F(vp[d]) * flip);
+  }
+}
+
 template <class L, class T>
 __global__ __launch_bounds__(256) void k_owner(ParticleArgs a) {
   constexpr int D = L::D;
@@ -171,7 +208,8 @@ __global__ __launch_bounds__(256) void k_gather(ParticleArgs a) {
   const T eps = T(a.eps);
   const T one = T(1.0);
   // ts = 1/sub_steps, u_p = (X − X_old)/ts (psm.hpp:160-172)
-  const T ts = one / T(a.sub_steps);
+  // PSM divides by sub_steps as given (psm.hpp:160-172); HLBM takes 0 as 1
+  const T ts = one / T(a.hlbm && a.sub_steps == 0 ? uint64_t(1) : a.sub_steps);
   T vp[3] = {T(0), T(0), T(0)};
   for (int d = 0; d < D; ++d) vp[d] = (P[PF_POS + d] - P[PF_POS_OLD + d]) / ts;
   F* por = static_cast<F*>(a.por);
@@ -203,7 +241,8 @@ __global__ __launch_bounds__(256) void k_gather(ParticleArgs a) {
 #pragma unroll
       for (int d = 0; d < D; ++d) mom[d] = mom[d] + T(cdir<L>(i, d)) * added;
     }
-    const T flip = one - B;
+    // PSM scales the exchange by the solid fraction; HLBM keeps it raw (hlbm.hpp:229-231)
+    const T flip = a.hlbm ? one : one - B;
     T fc[3] = {T(0), T(0), T(0)};
     for (int d = 0; d < D; ++d) fc[d] = mom[d] * flip;
     if (a.owner[2 * c + 1] == p + 1)
@@ -249,7 +288,7 @@ __global__ void k_finalize(ParticleArgs a) {
     P[PF_FORCE + d] = d < D ? T(s[d]) : T(0);
     P[PF_TORQUE + d] = T(0);
   }
-  if (a.psm_torque) {
+  if (a.psm_torque || a.hlbm) {
     if (D == 2) P[PF_TORQUE] = T(s[3]);
     else for (int d = 0; d < 3; ++d) P[PF_TORQUE + d] = T(s[3 + d]);
   }
@@ -551,6 +590,7 @@ static hipError_t couple_t(const ParticleArgs& a, hipStream_t s) {
   const bool scatter = a.porosity_mode == LBM_POROSITY_SCATTER;
   hipLaunchKernelGGL(k_clear, dim3((N + 255) / 256), dim3(256), 0, s, fdbl ? nullptr : (float*)a.por,
                      fdbl ? (double*)a.por : nullptr, a.owner, N, scatter ? 1 : 0);
+  if (a.hlbm) hipLaunchKernelGGL((k_hlbm_reset<F>), dim3((N + 255) / 256), dim3(256), 0, s, (F*)a.pn, (F*)a.pd, N, L::D);
   const dim3 pg(a.n * a.blocks_per_particle);
   hipLaunchKernelGGL((k_owner<L, T>), pg, dim3(256), 0, s, a);
   hipLaunchKernelGGL((k_gather<L, S, T, F>), pg, dim3(256), 0, s, a);
@@ -591,6 +631,20 @@ hipError_t launch_psm_couple(int lattice, int storage, const ParticleArgs& a, hi
     case LBM_D3Q27: return couple_l<D3Q27>(storage, a, s);
   }
   return hipErrorInvalidValue;
+}
+
+template <class T, class F>
+static hipError_t stamp_t(int dims, const ParticleArgs& a, hipStream_t s) {
+  const dim3 pg(a.n * a.blocks_per_particle);
+  if (dims == 2) hipLaunchKernelGGL((k_hlbm_stamp<T, F, 2>), pg, dim3(256), 0, s, a);
+  else hipLaunchKernelGGL((k_hlbm_stamp<T, F, 3>), pg, dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_hlbm_stamp(int dims, int storage, const ParticleArgs& a, hipStream_t s) {
+  if (a.n == 0) return hipSuccess;
+  if (storage == LBM_F64) return stamp_t<double, double>(dims, a, s);
+  return a.state64 ? stamp_t<double, float>(dims, a, s) : stamp_t<float, float>(dims, a, s);
 }
 
 hipError_t launch_dem_contact(int dims, int storage, const ParticleArgs& a, hipStream_t s) {

@@ -170,6 +170,25 @@ __device__ __forceinline__ void apply_op(T (&f)[L::Q], const OpDev* op, const St
 #pragma unroll
         for (int d = 0; d < L::D; ++d) F[d] = (use_cf ? cf[size_t(d) * a.N + cell] : T(0)) + T(op->vec[d]);
         bgk_guo<L, T, WB>(f, rho, u, om, F, op->force_mask);
+      } else if (coll == LBM_COLL_HLBM) {
+        // hlbm.hpp:62-105: velocity u*B + N(1-B)/D (N when D = 0); the macros
+        // receive that velocity, written here after the plain moments
+        const T Bp = static_cast<const T*>(a.por)[cell];
+        const T flip = T(1.0) - Bp;
+        const T Dw = static_cast<const T*>(a.pd)[cell];
+        const T* pn = static_cast<const T*>(a.pn);
+#pragma unroll
+        for (int d = 0; d < L::D; ++d) {
+          const T Nd = pn[size_t(d) * a.N + cell];
+          const T term = (Dw > T(0)) ? (Nd * flip) / Dw : Nd;
+          u[d] = u[d] * Bp + term;
+        }
+        if (a.write_macros) {
+          T* um = static_cast<T*>(a.u);
+#pragma unroll
+          for (int d = 0; d < L::D; ++d) um[size_t(d) * a.N + cell] = u[d];
+        }
+        bgk<L, T, WB>(f, rho, u, om);
       } else if (coll == LBM_COLL_PSM) {
         psm<L, T, WB>(f, rho, u, om, B);
         if (op->flags & (LBM_OPF_PSM_CELL_FORCE | LBM_OPF_PSM_FORCE_GATHER)) {

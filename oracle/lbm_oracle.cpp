@@ -15,6 +15,9 @@
 //   velocity (D2Q9)       modules/core/c++/boundary.hpp:32-805
 //   PsmReset / Psm / PsmOneParticleGather   modules/core/c++/psm.hpp:20-446
 //   PSM (cell force)      modules/core/c++/component/psm.hpp:16-80
+//   HlbmReset / Hlbm / HlbmOneParticleGather / HlbmOneParticleStamp
+//                         modules/core/c++/hlbm.hpp:28-49, 62-105, 111-256, 264-308;
+//                         driver order examples/poiseulle_particles_2d_gpu/step.hpp:21-43
 //   ForceGather           modules/core/c++/momentum_gather.hpp:15-52
 //   OneParticleAt         modules/core/c++/particle/particle_opa.hpp:14-46
 //   particle porosity     modules/core/c++/particle/porosity.hpp:56-89
@@ -506,6 +509,7 @@ struct SimBase {
   virtual void dem_contact() = 0;
   virtual void integrate_all() = 0;
   virtual uint64_t overflow() const = 0;
+  virtual void get_hlbm(double*, double*) const = 0;
 };
 
 template <class T, class L>
@@ -517,6 +521,7 @@ struct Sim final : SimBase {
   std::vector<T> buf[2];  // [0] "dfs_old", [1] "dfs"
   std::vector<uint8_t> info;
   std::vector<T> rho, vel, force, por, bc_vel;
+  std::vector<T> pN, pD;  // HLBM particle velocity sums N (D per cell) and weights D
   uint32_t periodic = 0;
   lbm_op_desc ops[256];
   bool has_op[256];
@@ -538,6 +543,8 @@ struct Sim final : SimBase {
     vel.assign(N * D, T(0));
     force.assign(N * D, T(0));
     por.assign(N, T(1));
+    pN.assign(N * D, T(0));
+    pD.assign(N, T(0));
     bc_vel.assign(N * D, T(0));
     std::memset(ops, 0, sizeof(ops));
     std::memset(has_op, 0, sizeof(has_op));
@@ -860,6 +867,23 @@ struct Sim final : SimBase {
         }
         break;
       }
+      case LBM_COLL_HLBM: {  // hlbm.hpp:62-105
+        T r, u[3], eq[27];
+        rho_u<T, L>(f, r, u);
+        const T B = por[c];
+        const T flip = T(T(1.0) - B);
+        const T Dw = pD[c];
+        for (int d = 0; d < D; ++d) {
+          const T Nd = pN[c * D + d];
+          const T term = (Dw > 0.0) ? T(T(Nd * flip) / Dw) : Nd;
+          u[d] = T(T(u[d] * B) + term);
+        }
+        rho[c] = r;
+        for (int d = 0; d < D; ++d) vel[c * D + d] = u[d];
+        equilibrium<T, L>(r, u, eq);
+        for (int i = 0; i < Q; ++i) f[i] = T(f[i] + T(omega * T(eq[i] - f[i])));
+        break;
+      }
       default:
         break;
     }
@@ -943,6 +967,90 @@ struct Sim final : SimBase {
     ABS[p] = abs64;
   }
 
+  // HlbmOneParticleGather (hlbm.hpp:111-256) for particle p: porosity
+  // assigned over the AABB, the raw momentum exchange as the cell force (no
+  // solid-fraction factor), force and torque sums subtracted from the body.
+  void gather_hlbm(uint32_t p, const T* a) {
+    Part<T>& P = parts[p];
+    const T one = T(1.0);
+    uint64_t steps = pp.sub_steps == 0 ? 1 : pp.sub_steps;
+    T ts = T(one / static_cast<T>(steps));
+    T vp[3];
+    for (int d = 0; d < D; ++d) vp[d] = T(T(P.pos[d] - P.pos_old[d]) / ts);
+    uint64_t lo[3] = {0, 0, 0}, hi[3] = {1, 1, 1};
+    aabb<T, D>(P.pos, P.radius, n, lo, hi);
+    T fp[3] = {T(0), T(0), T(0)}, tp[3] = {T(0), T(0), T(0)};
+    double f64[3] = {0, 0, 0}, t64[3] = {0, 0, 0}, abs64 = 0;
+    const T eps = T(1.5f);  // hlbm.hpp:173
+    uint64_t x[3] = {0, 0, 0};
+    for (x[0] = lo[0]; x[0] < hi[0]; ++x[0])
+      for (x[1] = lo[1]; x[1] < hi[1]; ++x[1])
+        for (x[2] = (D == 3 ? lo[2] : 0); x[2] < (D == 3 ? hi[2] : 1); ++x[2]) {
+          const uint64_t c = cell(x);
+          T rel[3];
+          for (int d = 0; d < D; ++d) rel[d] = T(static_cast<T>(x[d]) - P.pos[d]);
+          const T B = porosity<T, D>(rel, P.radius, eps);
+          por[c] = B;
+          if (B >= 1.0f) continue;
+          const T* f = a + gcell(x) * Q;
+          T mom[3] = {T(0.0), T(0.0), T(0.0)};
+          for (int i = 1; i < Q; ++i) {  // i = 0 adds e_0 = 0
+            const int o = L::opp[i];
+            T e[3];
+            for (int d = 0; d < D; ++d) e[d] = T(L::c[i][d]);
+            T upe = T(0);
+            for (int d = 0; d < D; ++d) upe = T(upe + T(e[d] * vp[d]));
+            uint64_t nb[3];
+            T fo = neighbour(x, L::c[i], +1, nb) ? a[gcell(nb) * Q + o] : T(0);
+            T added = T(T(f[i] * T(T(1) - upe)) + T(fo * T(T(1) + upe)));
+            for (int d = 0; d < D; ++d) mom[d] = T(mom[d] + T(e[d] * added));
+          }
+          for (int d = 0; d < D; ++d) {
+            force[c * D + d] = mom[d];
+            fp[d] = T(fp[d] + mom[d]);
+            f64[d] -= static_cast<double>(mom[d]);
+            abs64 += std::fabs(static_cast<double>(mom[d]));
+          }
+          T tq[3] = {T(0), T(0), T(0)};
+          cross_rf<T, D>(rel, mom, tq);
+          for (int d = 0; d < (D == 2 ? 1 : 3); ++d) {
+            tp[d] = T(tp[d] + tq[d]);
+            t64[d] -= static_cast<double>(tq[d]);
+          }
+        }
+    for (int d = 0; d < D; ++d) P.force[d] = T(P.force[d] - fp[d]);
+    for (int d = 0; d < (D == 2 ? 1 : 3); ++d) P.torque[d] = T(P.torque[d] - tp[d]);
+    for (int d = 0; d < 3; ++d) {
+      F64[p * 3 + d] = f64[d];
+      T64[p * 3 + d] = t64[d];
+    }
+    ABS[p] = abs64;
+  }
+
+  // HlbmOneParticleStamp (hlbm.hpp:264-308): after the move, every cell of
+  // the post-move AABB gains D += 1 - porosity and N += u_p (1 - porosity).
+  void stamp_hlbm() {
+    const T one = T(1.0);
+    uint64_t steps = pp.sub_steps == 0 ? 1 : pp.sub_steps;
+    const T ts = T(one / static_cast<T>(steps));
+    for (uint32_t p = 0; p < parts.size(); ++p) {
+      const Part<T>& P = parts[p];
+      T vp[3];
+      for (int d = 0; d < D; ++d) vp[d] = T(T(P.pos[d] - P.pos_old[d]) / ts);
+      uint64_t lo[3] = {0, 0, 0}, hi[3] = {1, 1, 1};
+      aabb<T, D>(P.pos, P.radius, n, lo, hi);
+      uint64_t x[3] = {0, 0, 0};
+      for (x[0] = lo[0]; x[0] < hi[0]; ++x[0])
+        for (x[1] = lo[1]; x[1] < hi[1]; ++x[1])
+          for (x[2] = (D == 3 ? lo[2] : 0); x[2] < (D == 3 ? hi[2] : 1); ++x[2]) {
+            const uint64_t c = cell(x);
+            const T flip = T(one - por[c]);
+            pD[c] = T(pD[c] + flip);
+            for (int d = 0; d < D; ++d) pN[c * D + d] = T(pN[c * D + d] + T(vp[d] * flip));
+          }
+    }
+  }
+
   int particles_set(const lbm_particle* in, uint32_t cnt, const lbm_particle_params* prm) override {
     parts.resize(cnt);
     pp = *prm;
@@ -993,6 +1101,19 @@ struct Sim final : SimBase {
   // in index order, then (product mode) the cell_porosity composition.
   void couple(uint64_t t) {
     const T* a = A(t);
+    if (pp.coupling == LBM_COUPLING_HLBM) {
+      // HlbmReset (hlbm.hpp:28-49), ParticleForceReset, gathers in index order
+      std::fill(por.begin(), por.end(), T(1.0));
+      std::fill(pN.begin(), pN.end(), T(0.0));
+      std::fill(pD.begin(), pD.end(), T(0.0));
+      for (auto& q : parts)
+        for (int d = 0; d < 3; ++d) {
+          q.force[d] = T(0);
+          q.torque[d] = T(0);
+        }
+      for (uint32_t p = 0; p < parts.size(); ++p) gather_hlbm(p, a);
+      return;
+    }
     std::fill(por.begin(), por.end(), T(1.0));  // PsmReset (psm.hpp:20-32)
     const bool product = pp.porosity_mode == LBM_POROSITY_PRODUCT;
     for (uint32_t p = 0; p < parts.size(); ++p) {  // reset_particle_accumulators
@@ -1039,6 +1160,13 @@ struct Sim final : SimBase {
     for (auto& p : parts) integrate<T, D>(p, g, pp.sub_steps);
   }
   uint64_t overflow() const override { return cl.overflow; }
+  void get_hlbm(double* nsum, double* dsum) const override {
+    for (uint64_t c = 0; c < N; ++c) {
+      if (dsum) dsum[c] = pD[c];
+      for (int d = 0; d < D; ++d)
+        if (nsum) nsum[uint64_t(d) * N + c] = pN[c * D + d];
+    }
+  }
 
   void step(uint64_t t0, uint64_t ns, int threads) override {
 #ifdef _OPENMP
@@ -1051,6 +1179,7 @@ struct Sim final : SimBase {
         couple(t);
         dem_contact();
         integrate_all();
+        if (pp.coupling == LBM_COUPLING_HLBM) stamp_hlbm();
       }
       T* a = A(t);
       T* b = B(t);
@@ -1157,7 +1286,32 @@ static void cell_op_t(const lbm_op_desc* op, uint64_t t, double* f, const double
     if (force_out) force_out[d] = s.force[d];
   }
 }
+template <class T, class L>
+static void cell_hlbm_t(const lbm_op_desc* op, double* f, double porosity_in, const double* nsum, double dsum,
+                        double* rho_out, double* u_out) {
+  uint64_t ext[3] = {1, 1, 1};
+  Sim<T, L> s(ext);
+  uint8_t fl = 2;
+  s.set_flags(&fl);
+  s.set_operator(2, op);
+  T* a = s.A(0);
+  uint64_t x[3] = {0, 0, 0};
+  T* ff = a + s.gcell(x) * L::Q;
+  for (int i = 0; i < L::Q; ++i) ff[i] = T(f[i]);
+  for (int d = 0; d < L::D; ++d) s.pN[d] = T(nsum[d]);
+  s.pD[0] = T(dsum);
+  s.por[0] = T(porosity_in);
+  s.collide_cell(0, 0, a);
+  for (int i = 0; i < L::Q; ++i) f[i] = ff[i];
+  if (rho_out) *rho_out = s.rho[0];
+  for (int d = 0; d < L::D; ++d)
+    if (u_out) u_out[d] = s.vel[d];
+}
 extern "C" {
+void orc_cell_hlbm(int lat, int st, const lbm_op_desc* op, double* f, double porosity_in, const double* nsum,
+                   double dsum, double* rho_out, double* u_out) {
+  ORC_DISPATCH(lat, st, cell_hlbm_t, op, f, porosity_in, nsum, dsum, rho_out, u_out);
+}
 void orc_cell_op(int lat, int st, const lbm_op_desc* op, uint64_t t, double* f,
                  const double* cell_force, const double* bc_u, double porosity_in,
                  double* rho_out, double* u_out, double* force_out) {
@@ -1251,6 +1405,7 @@ void orc_psm_couple(void* s, uint64_t t, double* f64, double* t64, double* abs_s
 void orc_dem_contact(void* s) { static_cast<SimBase*>(s)->dem_contact(); }
 void orc_particle_integrate(void* s) { static_cast<SimBase*>(s)->integrate_all(); }
 uint64_t orc_cell_list_overflow(void* s) { return static_cast<SimBase*>(s)->overflow(); }
+void orc_sim_get_hlbm(void* s, double* nsum, double* dsum) { static_cast<SimBase*>(s)->get_hlbm(nsum, dsum); }
 
 // narrow_phase<Spheroid>::contact on a canonical (lo, hi) pair.
 int orc_narrow(int D, int st, const double* pos_lo, double r_lo, const double* pos_hi, double r_hi,

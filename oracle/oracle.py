@@ -59,6 +59,9 @@ def lib():
         L.orc_dem_contact.argtypes = [vp]
         L.orc_particle_integrate.argtypes = [vp]
         L.orc_cell_list_overflow.argtypes = [vp]
+        L.orc_cell_hlbm.argtypes = [C.c_int, C.c_int, C.POINTER(abi.OpDesc), dp, C.c_double, dp, C.c_double,
+                                    dp, dp]
+        L.orc_sim_get_hlbm.argtypes = [vp, dp, dp]
         L.orc_cell_list_overflow.restype = C.c_uint64
         L.orc_max_threads.restype = C.c_int
         L.orc_narrow.argtypes = [C.c_int, C.c_int, dp, C.c_double, dp, C.c_double, dp, dp, dp]
@@ -115,6 +118,20 @@ def cell_op(lat, st, op, f, t=0, cell_force=None, bc_u=None, porosity=1.0):
     lib().orc_cell_op(lat, st, C.byref(op), t, _dp(ff), _dp(cf), _dp(bu), porosity,
                       _dp(r), _dp(u), _dp(fo))
     return ff[:Q], r[0], u[:D], fo[:D]
+
+
+def cell_hlbm(lat, st, op, f, porosity, nsum, dsum):
+    """The HLBM collision (hlbm.hpp:62-105) on one cell with given porosity
+    and particle sums N (D values) and D."""
+    D, Q = abi.LATTICE_DQ[lat]
+    ff = np.zeros(27)
+    ff[:Q] = f
+    nn = np.zeros(3)
+    nn[: len(nsum)] = nsum
+    r = np.zeros(1)
+    u = np.zeros(3)
+    lib().orc_cell_hlbm(lat, st, C.byref(op), _dp(ff), porosity, _dp(nn), dsum, _dp(r), _dp(u))
+    return ff[:Q], r[0], u[:D]
 
 
 def porosity(D, st, rel, r, eps):
@@ -267,3 +284,10 @@ class Sim:
 
     def overflow(self):
         return lib().orc_cell_list_overflow(self._h)
+
+    def get_hlbm(self):
+        """HLBM particle sums: N (D x N) and D (N)."""
+        n = np.zeros(self.D * self.N)
+        d = np.zeros(self.N)
+        lib().orc_sim_get_hlbm(self._h, _dp(n), _dp(d))
+        return n.reshape(self.D, self.N), d

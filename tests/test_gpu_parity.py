@@ -426,3 +426,64 @@ def test_static_disc_psm_driver(hip, orc, st):
     assert np.max(np.abs(ug - uo)) <= tol * np.max(np.abs(uo))
     ffg, ffo = g.get_force_field(), o.get_force_field()
     assert np.max(np.abs(ffg - ffo)) <= tol * max(np.max(np.abs(ffo)), 1e-300)
+
+
+@pytest.mark.parametrize("st", [abi.F64, abi.F32])
+def test_hlbm_collision_on_porosity_field_bitwise(hip, orc, st):
+    """The HLBM collision (hlbm.hpp:62-105) with a given porosity field and no
+    particles (N = D = 0, so u -> u*B): bitwise against the oracle, both AA
+    phases, inlet/outlet Zou/He + HLBM as in the HLBM drivers."""
+    ext = (40, 18)
+    g, o = _pair(hip, orc, abi.D2Q9, st, ext)
+    flags = channel_flags_2d(*ext)
+    ops = {1: abi.op_desc(boundary=abi.BC_BOUNCE_BACK), 2: abi.op_desc(collision=abi.COLL_HLBM, tau=0.8),
+           3: abi.op_desc(boundary=abi.BC_ZOUHE_PRESSURE_X, collision=abi.COLL_HLBM, east=1, tau=0.8,
+                          ramp=(1.0, 0.01 / 16, 1.01, 16)),
+           4: abi.op_desc(boundary=abi.BC_ZOUHE_PRESSURE_X, collision=abi.COLL_HLBM, east=0, tau=0.8, rho=1.0)}
+    rng = np.random.default_rng(41)
+    por = np.clip(rng.uniform(0.2, 1.4, ext[0] * ext[1]), 0, 1)
+    _setup((g, o), flags, ops, por=por)
+    for n in (1, 2, 37):
+        g.step(n)
+        o.step(n)
+        _assert_same(g, o, exact=True)
+
+
+@pytest.mark.parametrize("st", [abi.F64, abi.F32])
+def test_hlbm_particle_laden_run(hip, orc, st):
+    """HLBM particle coupling (HlbmReset, HlbmOneParticleGather, integrate,
+    HlbmOneParticleStamp, HLBM collision; examples/poiseulle_particles_2d_gpu/
+    step.hpp:21-93) with 6 discs for 60 steps against the oracle."""
+    ext = (96, 48)
+    g, o = _pair(hip, orc, abi.D2Q9, st, ext)
+    flags = channel_flags_2d(*ext)
+    ops = {1: abi.op_desc(boundary=abi.BC_BOUNCE_BACK), 2: abi.op_desc(collision=abi.COLL_HLBM, tau=0.8),
+           3: abi.op_desc(boundary=abi.BC_ZOUHE_PRESSURE_X, collision=abi.COLL_HLBM, east=1, tau=0.8,
+                          ramp=(1.0, 0.01 / 16, 1.01, 16)),
+           4: abi.op_desc(boundary=abi.BC_ZOUHE_PRESSURE_X, collision=abi.COLL_HLBM, east=0, tau=0.8, rho=1.0)}
+    _setup((g, o), flags, ops)
+    discs = seeded_discs(6, ext, 4.0, 7.0, 17, margin=4.0, gap=4.0)
+    parts = [make_particle([x, y], r, 1.5, 2, st, orc=orc) for (x, y, r) in discs]
+    prm = abi.particle_params(k_n=0.0, zeta_n=0.0, walls=False, sub_steps=16,
+                              body_acceleration=(0.0, -1e-4), max_radius=7.0, coupling=abi.COUPLING_HLBM)
+    for s in (g, o):
+        s.particles_set(parts, prm)
+    g.step(60)
+    o.step(60)
+    pg, po = g.particles_get(), o.particles_get()
+    moved = max(abs(a.position[d] - b_[d]) for a, b_ in zip(po, [(x, y) for (x, y, _) in discs])
+                for d in range(2))
+    assert moved > 1e-4
+    # FP64 force/torque sums on the GPU, serial T sums in the oracle: FP64 runs
+    # agree to the summation order, FP32 runs to FP32 rounding of the feedback
+    tol = 1e-11 if st == abi.F64 else 1e-4
+    for a, b in zip(pg, po):
+        for d in range(2):
+            assert abs(a.position[d] - b.position[d]) <= tol * max(abs(b.position[d]), 1.0)
+        assert abs(a.torque[0] - b.torque[0]) <= (1e-9 if st == abi.F64 else 1e-3) * max(abs(b.torque[0]), 1e-3)
+    rg, ug = g.get_macros()
+    ro, uo = o.get_macros()
+    ftol = 1e-12 if st == abi.F64 else 1e-4
+    assert np.max(np.abs(rg - ro)) <= ftol
+    assert np.max(np.abs(ug - uo)) <= ftol * np.max(np.abs(uo))
+    assert np.max(np.abs(g.get_porosity() - o.get_porosity())) <= (1e-13 if st == abi.F64 else 1e-6)

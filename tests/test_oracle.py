@@ -651,3 +651,111 @@ def test_mass_conservation_closed_box(orc):
         m0 = s.get_populations().sum()
         s.step(20)
         assert abs(s.get_populations().sum() - m0) < 1e-12 * m0
+
+
+# ------------------------------------------------------------------ HLBM
+@pytest.mark.parametrize("lat", [abi.D2Q9, abi.D3Q19, abi.D3Q27])
+def test_hlbm_collision_limits_and_velocity(orc, lat):
+    """hlbm.hpp:62-105: with porosity 1 (and N = D = 0) the HLBM collision is
+    BGK bit for bit; otherwise it is BGK towards eq(rho, u*B + N(1-B)/D)
+    (N itself when D = 0), and that velocity is the one written to the macros."""
+    Dd, Q, c, w, opp = orc.lattice(lat)
+    rng = np.random.default_rng(31)
+    f = _rand_f(lat, rng)
+    op_h = abi.op_desc(collision=abi.COLL_HLBM, tau=0.8)
+    bgk, rb, ub, _ = orc.cell_op(lat, F64, abi.op_desc(collision=abi.COLL_BGK, tau=0.8), f)
+    h1, rh, uh = orc.cell_hlbm(lat, F64, op_h, f, 1.0, [0.0] * Dd, 0.0)
+    assert (h1 == bgk).all() and rh == rb and (uh == ub).all()
+    rho = f.sum()
+    uf = (c[:, :Dd].T @ f) / rho
+    om = 1.0 / 0.8
+    for B, Nv, Dw in ((0.0, [0.02, -0.01, 0.005], 0.7), (0.35, [0.013, 0.004, -0.02], 1.6),
+                      (0.6, [0.01, 0.02, 0.0], 0.0)):
+        Nv = np.array(Nv[:Dd])
+        h, r, u = orc.cell_hlbm(lat, F64, op_h, f, B, Nv, Dw)
+        want_u = uf * B + ((Nv * (1 - B)) / Dw if Dw > 0 else Nv)
+        assert np.allclose(u, want_u, rtol=1e-13, atol=1e-16)
+        eq = orc.equilibrium(lat, F64, r, u)
+        assert np.allclose(h, f + om * (eq - f), rtol=1e-13, atol=1e-16)
+
+
+def test_hlbm_gather_brute_force_newton_torque(orc):
+    """HlbmOneParticleGather (hlbm.hpp:111-256) on a tiny lattice against a
+    direct numpy evaluation: cell force = the momentum exchange with no
+    solid-fraction factor, body force = -sum, body torque = -sum r x f,
+    porosity assigned over the AABB."""
+    lat = abi.D2Q9
+    D, Q, c, w, opp = orc.lattice(lat)
+    ext = (24, 20)
+    N = ext[0] * ext[1]
+    rng = np.random.default_rng(32)
+    s = orc.Sim(lat, F64, ext)
+    s.set_flags(np.full(N, 2, dtype=np.uint8))
+    f = np.array([orc.equilibrium(lat, F64, 1.0, [0.02, -0.01]) for _ in range(N)]).T
+    f = f * (1 + 0.05 * rng.uniform(-1, 1, size=f.shape))
+    s.set_populations(f)
+    pos, vel, R = [11.3, 9.6], [0.01, -0.004], 4.5
+    p = make_particle(pos, R, 1.5, 2, F64, vel=[v / 4 for v in vel], orc=orc)
+    s.particles_set([p], abi.particle_params(sub_steps=4, eps=1.5, max_radius=R, coupling=abi.COUPLING_HLBM))
+    s.psm_couple()
+    q = s.particles_get()[0]
+    fs = f.reshape(Q, ext[1], ext[0])
+    up = np.array(vel)
+    a = [int(max(pos[d] - R, 0)) for d in range(2)]
+    b = [min(int(pos[d] + R + 1), ext[d]) for d in range(2)]
+    Fp, Tp = np.zeros(2), 0.0
+    por_ref = np.ones((ext[1], ext[0]))
+    cf_ref = np.zeros((2, ext[1], ext[0]))
+    for x in range(a[0], b[0]):
+        for y in range(a[1], b[1]):
+            B = orc.porosity(2, F64, [x - pos[0], y - pos[1]], R, 1.5)
+            por_ref[y, x] = B
+            if B >= 1:
+                continue
+            mom = np.zeros(2)
+            for i in range(1, Q):
+                xn, yn = x + c[i, 0], y + c[i, 1]
+                fo = fs[opp[i], yn, xn] if (0 <= xn < ext[0] and 0 <= yn < ext[1]) else 0.0
+                cu = c[i, :2] @ up
+                mom += c[i, :2] * (fs[i, y, x] * (1 - cu) + fo * (1 + cu))
+            cf_ref[:, y, x] = mom
+            Fp -= mom
+            Tp -= (x - pos[0]) * mom[1] - (y - pos[1]) * mom[0]
+    assert np.allclose([q.force[0], q.force[1]], Fp, rtol=1e-12, atol=1e-15)
+    assert np.isclose(q.torque[0], Tp, rtol=1e-11, atol=1e-15)
+    assert (s.get_porosity().reshape(ext[1], ext[0]) == por_ref).all()
+    ff = s.get_force_field().reshape(2, ext[1], ext[0])
+    assert np.allclose(ff, cf_ref, rtol=1e-13, atol=1e-16)
+    assert np.allclose(ff.reshape(2, -1).sum(axis=1), -Fp, rtol=1e-12, atol=1e-15)
+
+
+def test_hlbm_stamp_after_move(orc):
+    """HlbmOneParticleStamp (hlbm.hpp:264-308): after one full step with one
+    moving disc, D = 1 - porosity and N = u_p (1 - porosity) on the post-move
+    AABB (porosity from the pre-move gather), zero elsewhere."""
+    lat = abi.D2Q9
+    ext = (30, 24)
+    N = ext[0] * ext[1]
+    s = orc.Sim(lat, F64, ext)
+    s.set_flags(np.full(N, 2, dtype=np.uint8))
+    s.set_operator(2, abi.op_desc(collision=abi.COLL_HLBM, tau=0.8))
+    s.init_equilibrium(1.0, (0.0, 0.0))
+    pos, R = [14.2, 11.7], 5.0
+    p = make_particle(pos, R, 1.5, 2, F64, vel=[0.003 / 8, -0.002 / 8], orc=orc)
+    prm = abi.particle_params(sub_steps=8, eps=1.5, max_radius=R, coupling=abi.COUPLING_HLBM,
+                              body_acceleration=(0.0, -1e-4))
+    s.particles_set([p], prm)
+    s.step(1)
+    q = s.particles_get()[0]
+    por = s.get_porosity().reshape(ext[1], ext[0])
+    nsum, dsum = s.get_hlbm()
+    dsum = dsum.reshape(ext[1], ext[0])
+    nsum = nsum.reshape(2, ext[1], ext[0])
+    up = np.array([(q.position[d] - q.position_old[d]) * 8 for d in range(2)])
+    a = [int(max(q.position[d] - R, 0)) for d in range(2)]
+    b = [min(int(q.position[d] + R + 1), ext[d]) for d in range(2)]
+    want_d = np.zeros_like(dsum)
+    want_d[a[1]:b[1], a[0]:b[0]] = 1 - por[a[1]:b[1], a[0]:b[0]]
+    assert np.array_equal(dsum, want_d)
+    assert np.allclose(nsum, up[:, None, None] * want_d[None], rtol=1e-15, atol=0)
+    assert dsum.max() > 0.9
