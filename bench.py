@@ -122,8 +122,10 @@ def bench_c3(args, storage):
     steps = args.steps if args.steps != 1000 else 2000
 
     hlbm = args.coupling == "hlbm"
-    fluid = abi.COLL_HLBM if hlbm else abi.COLL_PSM
+    fplbm = args.coupling == "fplbm"
+    fluid = {"psm": abi.COLL_PSM, "hlbm": abi.COLL_HLBM, "fplbm": abi.COLL_FPLBM}[args.coupling]
     edge = abi.COLL_HLBM if hlbm else abi.COLL_BGK
+    coupling = {"psm": abi.COUPLING_PSM, "hlbm": abi.COUPLING_HLBM, "fplbm": abi.COUPLING_FPLBM}[args.coupling]
 
     def run(st, s64=False):
         lat = Lattice(abi.D2Q9, st, (nx, ny))
@@ -147,7 +149,7 @@ def bench_c3(args, storage):
             parts.append(p)
         prm = abi.particle_params(k_n=1e3, zeta_n=restitution_to_zeta(0.9), walls=True, sub_steps=16,
                                   body_acceleration=(0.0, -1e-6), eps=1.5, max_radius=16.0, state_fp64=s64,
-                                  coupling=abi.COUPLING_HLBM if hlbm else abi.COUPLING_PSM)
+                                  coupling=coupling)
         lat.particles_set(parts, prm)
         lat.step(args.warmup, t0=0)
         lat.sync()
@@ -167,14 +169,14 @@ def bench_c3(args, storage):
     err_u64 = float(np.linalg.norm(u_s64 - u64) / np.linalg.norm(u64))
     err_x64 = float(np.max(np.linalg.norm(pos_s64 - pos64, axis=1)))
     sb = {abi.F32: 4, abi.F16S: 2, abi.BF16S: 2}[storage]
-    return {"metric": "MLUPS (D2Q9 %s particle-laden channel, full step incl. particles)" % ("HLBM" if hlbm else "PSM"),
+    return {"metric": "MLUPS (D2Q9 %s particle-laden channel, full step incl. particles)" % args.coupling.upper(),
             "value": round(nx * ny * steps / (ms * 1e3), 1), "unit": "MLUPS", "n_gpus": 1,
             "steps": steps, "warmup": args.warmup, "ms_per_step": round(ms / steps, 5),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": {abi.F32: "f32", abi.F16S: "f16-storage/f32", abi.BF16S: "bf16-storage/f32"}[storage],
             "data": "synthetic",
             "config": {"workload": "D2Q9 %s 2048x512, 200 discs r~U[8,16], rho_p 1.5, g 1e-6, "
-                                   "DEM k 1e3 e 0.9, tau 0.7" % ("HLBM" if hlbm else "PSM"),
+                                   "DEM k 1e3 e 0.9, tau 0.7" % args.coupling.upper(),
                        "error_vs_fp64": {"velocity_rel_l2": err_u, "trajectory_max_cells": err_x,
                                          "max_displacement_cells": disp},
                        "with_fp64_particle_state": {"velocity_rel_l2": err_u64, "trajectory_max_cells": err_x64,
@@ -373,7 +375,7 @@ def main():
     ap.add_argument("--train-max-steps", type=int, default=100000)
     ap.add_argument("--test-max-steps", type=int, default=400000)
     ap.add_argument("--members", type=int, default=256)
-    ap.add_argument("--coupling", default="psm", choices=["psm", "hlbm"],
+    ap.add_argument("--coupling", default="psm", choices=["psm", "hlbm", "fplbm"],
                     help="particle coupling of --config c3")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -35,6 +35,7 @@ struct BGK {};
 struct BGKGuo {};
 struct Psm {};
 struct Hlbm {};
+struct FpLbm {};
 struct BounceBack {};
 template <uint64_t I> struct AntiBounceBack {};
 template <bool East> struct ZouHePressureX {};
@@ -92,6 +93,20 @@ class component<T, Desc, cmpt::Hlbm> {
   explicit component(double relaxation) : tau_(relaxation) {}
   void fill(lbm_op_desc& d) const {
     d.collision = LBM_COLL_HLBM;
+    d.tau = tau_;
+  }
+ private:
+  double tau_;
+};
+
+// cmpt::FpLbm (fplbm.hpp:60-133): Guo forcing with the cell force field;
+// the particle side is lbm_particle_params::coupling = LBM_COUPLING_FPLBM.
+template <typename T, typename Desc>
+class component<T, Desc, cmpt::FpLbm> {
+ public:
+  explicit component(double relaxation) : tau_(relaxation) {}
+  void fill(lbm_op_desc& d) const {
+    d.collision = LBM_COLL_FPLBM;
     d.tau = tau_;
   }
  private:

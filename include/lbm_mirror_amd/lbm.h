@@ -77,12 +77,15 @@ typedef enum lbm_collision_kind {
   LBM_COLL_BGK = 1,      /* cmpt::BGK     collision.hpp:76-96            */
   LBM_COLL_BGK_GUO = 2,  /* cmpt::BGKGuo  collision.hpp:135-204          */
   LBM_COLL_PSM = 3,      /* cmpt::Psm     psm.hpp:58-89                  */
-  LBM_COLL_HLBM = 4      /* cmpt::Hlbm    hlbm.hpp:62-105                */
+  LBM_COLL_HLBM = 4,     /* cmpt::Hlbm    hlbm.hpp:62-105                */
+  LBM_COLL_FPLBM = 5     /* cmpt::FpLbm   fplbm.hpp:60-133: Guo forcing with the cell
+                            force field, u = m/rho; writes rho and the momentum m */
 } lbm_collision_kind;
 
 typedef enum lbm_particle_coupling {
   LBM_COUPLING_PSM = 0,  /* PsmReset + PsmOneParticleGather (psm.hpp:20-32,101-445)        */
-  LBM_COUPLING_HLBM = 1  /* HlbmReset + HlbmOneParticleGather/Stamp (hlbm.hpp:28-49,111-325) */
+  LBM_COUPLING_HLBM = 1, /* HlbmReset + HlbmOneParticleGather/Stamp (hlbm.hpp:28-49,111-325) */
+  LBM_COUPLING_FPLBM = 2 /* FpLbmReset + FpLbmOneParticle (fplbm.hpp:21-58,135-247)          */
 } lbm_particle_coupling;
 
 /* lbm_op_desc.flags */

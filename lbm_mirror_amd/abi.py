@@ -34,8 +34,8 @@ BC_NONE, BC_BOUNCE_BACK, BC_ANTI_BOUNCE_BACK = 0, 1, 2
 BC_ZOUHE_PRESSURE_X, BC_ZOUHE_VELOCITY_X, BC_EQUILIBRIUM = 3, 4, 5
 
 # lbm_collision_kind
-COLL_NONE, COLL_BGK, COLL_BGK_GUO, COLL_PSM, COLL_HLBM = 0, 1, 2, 3, 4
-COUPLING_PSM, COUPLING_HLBM = 0, 1
+COLL_NONE, COLL_BGK, COLL_BGK_GUO, COLL_PSM, COLL_HLBM, COLL_FPLBM = 0, 1, 2, 3, 4, 5
+COUPLING_PSM, COUPLING_HLBM, COUPLING_FPLBM = 0, 1, 2
 
 OPF_GUO_CELL_FORCE = 0x1
 OPF_PSM_STAMP_DISC = 0x2

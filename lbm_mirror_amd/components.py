@@ -47,6 +47,14 @@ class Hlbm:
     tau: float
 
 
+@dataclass
+class FpLbm:
+    """cmpt::FpLbm (fplbm.hpp:60-133): Guo forcing with the cell force field.
+    The particle side (FpLbmReset, FpLbmOneParticle) is selected with
+    ``abi.particle_params(coupling=abi.COUPLING_FPLBM)``."""
+    tau: float
+
+
 # ---- boundaries -------------------------------------------------------------
 @dataclass
 class BounceBack:
@@ -106,6 +114,8 @@ def combine(boundary=None, collision=None):
             flags |= abi.OPF_GUO_CELL_FORCE
     elif isinstance(collision, Hlbm):
         kw.update(collision=abi.COLL_HLBM, tau=collision.tau)
+    elif isinstance(collision, FpLbm):
+        kw.update(collision=abi.COLL_FPLBM, tau=collision.tau)
     elif isinstance(collision, Psm):
         kw.update(collision=abi.COLL_PSM, tau=collision.tau)
         if collision.stamp_disc is not None:

@@ -107,6 +107,9 @@ struct ParticleArgs {
   uint32_t cap;
   int32_t state64;  // particle records and arithmetic in FP64 whatever the storage
   int32_t hlbm;     // HLBM coupling: raw momentum as cell force, torque, stamp of N and D
+  int32_t fplbm;    // FPLBM coupling: penalty force (u_s - m)(1 - B), no torque
+  const uint8_t* cflags;  // cell flags (FPLBM: flag-2 cells take m from the populations)
+  const void* umac;       // D*N macro velocity field (FPLBM: m of the other cells)
   void* pn;         // D*N HLBM N sums (fluid field type)
   void* pd;         // N   HLBM D sums
 };

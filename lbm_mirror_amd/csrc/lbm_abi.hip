@@ -215,9 +215,13 @@ ParticleArgs particle_args(lbm_ctx* c) {
   for (int d = 0; d < 3; ++d) a.grid[d] = c->grid[d];
   a.cap = uint32_t(c->pp.cell_capacity);
   a.hlbm = c->pp.coupling == LBM_COUPLING_HLBM ? 1 : 0;
+  a.fplbm = c->pp.coupling == LBM_COUPLING_FPLBM ? 1 : 0;
   a.pn = c->pn;
   a.pd = c->pd;
-  if (a.hlbm) a.eps = to_T(pst, 1.5);  // HlbmOneParticleGather's fixed ramp width (hlbm.hpp:173)
+  a.cflags = c->flags;
+  a.umac = c->u;
+  // fixed ramp width of the HLBM gather and the FPLBM particle (hlbm.hpp:173, fplbm.hpp:193)
+  if (a.hlbm || a.fplbm) a.eps = to_T(pst, 1.5);
   return a;
 }
 
@@ -388,7 +392,7 @@ lbm_status lbm_set_operator(lbm_ctx* c, uint8_t flag, const lbm_op_desc* op) {
   if (flag >= kMaxOps) return fail(LBM_ERR_NOT_SUPPORTED, "operators on flags >= %d", kMaxOps);
   if (op->boundary < LBM_BC_NONE || op->boundary > LBM_BC_EQUILIBRIUM)
     return fail(LBM_ERR_INVALID_ARG, "unknown boundary kind %d", op->boundary);
-  if (op->collision < LBM_COLL_NONE || op->collision > LBM_COLL_HLBM)
+  if (op->collision < LBM_COLL_NONE || op->collision > LBM_COLL_FPLBM)
     return fail(LBM_ERR_INVALID_ARG, "unknown collision kind %d", op->collision);
   if (op->boundary == LBM_BC_ZOUHE_PRESSURE_X && c->lattice == LBM_D3Q19)
     return fail(LBM_ERR_NOT_SUPPORTED, "the reference has Zou/He pressure for D2Q9 and D3Q27 only");
@@ -423,7 +427,8 @@ lbm_status lbm_set_operator(lbm_ctx* c, uint8_t flag, const lbm_op_desc* op) {
     lbm_status s = ensure_field(c, &c->por, c->N, 1.0);
     if (s) return s;
   }
-  if ((o.coll == LBM_COLL_BGK_GUO && (o.flags & LBM_OPF_GUO_CELL_FORCE)) ||
+  if (o.coll == LBM_COLL_FPLBM ||
+      (o.coll == LBM_COLL_BGK_GUO && (o.flags & LBM_OPF_GUO_CELL_FORCE)) ||
       (o.coll == LBM_COLL_PSM && (o.flags & (LBM_OPF_PSM_CELL_FORCE | LBM_OPF_PSM_FORCE_GATHER)))) {
     lbm_status s = ensure_field(c, &c->force, size_t(c->D) * c->N, 0.0);
     if (s) return s;
@@ -601,7 +606,9 @@ lbm_status lbm_step(lbm_ctx* c, uint64_t t0, uint64_t nsteps) {
       s = particle_pass(c, t);
       if (s) return s;
     }
-    s = fluid_pass(c, t, k + 1 == nsteps);
+    // FPLBM reads the stored macro velocity of non-fluid cells every step
+    const bool every = c->np > 0 && c->pp.coupling == LBM_COUPLING_FPLBM;
+    s = fluid_pass(c, t, k + 1 == nsteps || every);
     if (s) return s;
   }
   HIP_TRY(hipEventRecord(c->ev1, c->stream));
@@ -659,10 +666,11 @@ lbm_status lbm_particles_set(lbm_ctx* c, const lbm_particle* p, uint32_t n,
     return fail(LBM_ERR_INVALID_ARG, "porosity mode %d", prm->porosity_mode);
   if (prm->cell_capacity <= 0 && prm->porosity_mode == LBM_POROSITY_PRODUCT)
     return fail(LBM_ERR_INVALID_ARG, "cell capacity must be positive");
-  if (prm->coupling != LBM_COUPLING_PSM && prm->coupling != LBM_COUPLING_HLBM)
+  if (prm->coupling != LBM_COUPLING_PSM && prm->coupling != LBM_COUPLING_HLBM &&
+      prm->coupling != LBM_COUPLING_FPLBM)
     return fail(LBM_ERR_INVALID_ARG, "coupling %d", prm->coupling);
-  if (prm->coupling == LBM_COUPLING_HLBM && prm->porosity_mode != LBM_POROSITY_SCATTER)
-    return fail(LBM_ERR_NOT_SUPPORTED, "HLBM assigns porosity per particle (scatter mode only)");
+  if (prm->coupling != LBM_COUPLING_PSM && prm->porosity_mode != LBM_POROSITY_SCATTER)
+    return fail(LBM_ERR_NOT_SUPPORTED, "HLBM and FPLBM assign porosity per particle (scatter mode only)");
   HIP_TRY(hipSetDevice(c->device));
   HIP_TRY(hipStreamSynchronize(c->stream));
   const int st = particle_storage(c->storage, *prm);

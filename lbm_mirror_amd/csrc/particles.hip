@@ -132,6 +132,13 @@ __global__ void k_clear(float* por_f, double* por_d, uint32_t* owner, uint32_t N
   owner[2 * c + 1] = 0;
 }
 
+// FpLbmReset's force = 0 over the lattice (fplbm.hpp:35-39)
+template <class F>
+__global__ void k_zero(F* v, size_t n) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = F(0);
+}
+
 // HlbmReset (hlbm.hpp:28-49): N = 0, D = 0 (porosity in k_clear)
 template <class F>
 __global__ void k_hlbm_reset(F* pn, F* pd, uint32_t N, int dims) {
@@ -210,8 +217,10 @@ __global__ __launch_bounds__(256) void k_gather(ParticleArgs a) {
   // ts = 1/sub_steps, u_p = (X − X_old)/ts (psm.hpp:160-172)
   // PSM divides by sub_steps as given (psm.hpp:160-172); HLBM takes 0 as 1
   const T ts = one / T(a.hlbm && a.sub_steps == 0 ? uint64_t(1) : a.sub_steps);
+  // body velocity: (X - X_old)/ts, or (X - X_old)*sub_steps for FPLBM (fplbm.hpp:198-200)
   T vp[3] = {T(0), T(0), T(0)};
-  for (int d = 0; d < D; ++d) vp[d] = (P[PF_POS + d] - P[PF_POS_OLD + d]) / ts;
+  for (int d = 0; d < D; ++d)
+    vp[d] = a.fplbm ? (P[PF_POS + d] - P[PF_POS_OLD + d]) * T(a.sub_steps) : (P[PF_POS + d] - P[PF_POS_OLD + d]) / ts;
   F* por = static_cast<F*>(a.por);
   F* fout = static_cast<F*>(a.force);
   const bool scatter = a.porosity_mode == LBM_POROSITY_SCATTER;
@@ -224,6 +233,32 @@ __global__ __launch_bounds__(256) void k_gather(ParticleArgs a) {
     const T B = porosity_at<T, D>(P, x, eps, rel);
     const uint32_t c = uint32_t(x[0]) + uint32_t(a.nx) * (uint32_t(x[1]) + uint32_t(a.ny) * uint32_t(x[2]));
     if (scatter && a.owner[2 * c] == p + 1) por[c] = F(B);
+    if (a.fplbm) {
+      // FpLbmOneParticle (fplbm.hpp:204-224): m of a flag-2 cell from its
+      // populations (FpLbmReset's compute_rho_momentum), else the stored macro
+      T mom[3] = {T(0), T(0), T(0)};
+      if (a.cflags[c] == 2) {
+#pragma unroll
+        for (int j = 1; j < Q; ++j) {
+          const T fj = pop_at<L, S, T>(a, j, x[0], x[1], x[2]);
+#pragma unroll
+          for (int d = 0; d < D; ++d) {
+            if (cdir<L>(j, d) == 1) mom[d] = mom[d] + fj;
+            if (cdir<L>(j, d) == -1) mom[d] = mom[d] - fj;
+          }
+        }
+      } else {
+        const F* um = static_cast<const F*>(a.umac);
+        for (int d = 0; d < D; ++d) mom[d] = T(um[size_t(d) * a.N + c]);
+      }
+      const T flip = one - B;
+      T fc[3] = {T(0), T(0), T(0)};
+      for (int d = 0; d < D; ++d) fc[d] = (vp[d] - mom[d]) * flip;
+      if (a.owner[2 * c] == p + 1)
+        for (int d = 0; d < D; ++d) fout[size_t(d) * a.N + c] = F(fc[d]);
+      for (int d = 0; d < D; ++d) acc[d] -= double(fc[d]);
+      continue;
+    }
     if (B >= T(1.0)) continue;
     T mom[3] = {T(0.0), T(0.0), T(0.0)};
 #pragma unroll
@@ -562,10 +597,13 @@ __global__ void k_integrate(ParticleArgs a) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= a.n) return;
   T* P = static_cast<T*>(a.parts) + size_t(s) * kPartFields;
+  // FpLbmOneParticle moves only for sub_steps > 0 and sets only the linear
+  // acceleration (fplbm.hpp:233-244)
+  if (a.fplbm && a.sub_steps == 0) return;
   const uint64_t steps = a.sub_steps == 0 ? 1 : a.sub_steps;
   const T ts = T(1) / T(steps);
   for (int k = 0; k < D; ++k) P[PF_ACC + k] = P[PF_FORCE + k] / P[PF_MASS] + T(a.g[k]);
-  if (D == 2) P[PF_ANG_ACC] = P[PF_TORQUE] / P[PF_INERTIA];
+  if (D == 2 && !a.fplbm) P[PF_ANG_ACC] = P[PF_TORQUE] / P[PF_INERTIA];
   constexpr int R = (D == 2) ? 1 : 3;
   for (uint64_t st = 0; st < steps; ++st) {
     const T ts2 = ts * ts;
@@ -591,6 +629,9 @@ static hipError_t couple_t(const ParticleArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_clear, dim3((N + 255) / 256), dim3(256), 0, s, fdbl ? nullptr : (float*)a.por,
                      fdbl ? (double*)a.por : nullptr, a.owner, N, scatter ? 1 : 0);
   if (a.hlbm) hipLaunchKernelGGL((k_hlbm_reset<F>), dim3((N + 255) / 256), dim3(256), 0, s, (F*)a.pn, (F*)a.pd, N, L::D);
+  if (a.fplbm)
+    hipLaunchKernelGGL((k_zero<F>), dim3((size_t(L::D) * N + 255) / 256), dim3(256), 0, s, (F*)a.force,
+                       size_t(L::D) * N);
   const dim3 pg(a.n * a.blocks_per_particle);
   hipLaunchKernelGGL((k_owner<L, T>), pg, dim3(256), 0, s, a);
   hipLaunchKernelGGL((k_gather<L, S, T, F>), pg, dim3(256), 0, s, a);

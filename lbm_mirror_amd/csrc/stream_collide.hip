@@ -170,6 +170,25 @@ __device__ __forceinline__ void apply_op(T (&f)[L::Q], const OpDev* op, const St
 #pragma unroll
         for (int d = 0; d < L::D; ++d) F[d] = (use_cf ? cf[size_t(d) * a.N + cell] : T(0)) + T(op->vec[d]);
         bgk_guo<L, T, WB>(f, rho, u, om, F, op->force_mask);
+      } else if (coll == LBM_COLL_FPLBM) {
+        // fplbm.hpp:60-133: BGKGuo's arithmetic with the cell force field and
+        // u = m/rho; the macros hold rho and the momentum (FpLbmReset)
+        T F[3] = {T(0), T(0), T(0)};
+        const T* cf = static_cast<const T*>(a.force);
+#pragma unroll
+        for (int d = 0; d < L::D; ++d) F[d] = cf[size_t(d) * a.N + cell];
+        if (a.write_macros) {
+          T m[3] = {T(0), T(0), T(0)};
+#pragma unroll
+          for (int j = 1; j < Q; ++j)
+#pragma unroll
+            for (int d = 0; d < L::D; ++d)
+              if (cdir<L>(j, d) != 0) m[d] = m[d] + T(cdir<L>(j, d)) * f[j];
+          T* um = static_cast<T*>(a.u);
+#pragma unroll
+          for (int d = 0; d < L::D; ++d) um[size_t(d) * a.N + cell] = m[d];
+        }
+        bgk_guo<L, T, WB>(f, rho, u, om, F, 7u);
       } else if (coll == LBM_COLL_HLBM) {
         // hlbm.hpp:62-105: velocity u*B + N(1-B)/D (N when D = 0); the macros
         // receive that velocity, written here after the plain moments

@@ -18,6 +18,9 @@
 //   HlbmReset / Hlbm / HlbmOneParticleGather / HlbmOneParticleStamp
 //                         modules/core/c++/hlbm.hpp:28-49, 62-105, 111-256, 264-308;
 //                         driver order examples/poiseulle_particles_2d_gpu/step.hpp:21-43
+//   FpLbmReset / FpLbm / FpLbmOneParticle   modules/core/c++/fplbm.hpp:21-58, 60-133, 135-247;
+//                         compute_rho_momentum  macroscopic.hpp:41-60;
+//                         driver order examples/moving_poiseulle_particles_2d_fplbm_gpu/sim.cpp:185-258
 //   ForceGather           modules/core/c++/momentum_gather.hpp:15-52
 //   OneParticleAt         modules/core/c++/particle/particle_opa.hpp:14-46
 //   particle porosity     modules/core/c++/particle/porosity.hpp:56-89
@@ -372,12 +375,13 @@ void wall_contacts(const Part<T>& me, Part<T>& self, const uint64_t* extent,
 
 // integrate_particle + verlet_step_lambda (integrate.hpp:45-83,
 // particle.hpp:171-211)
+// keep_ang: FpLbmOneParticle sets only the linear acceleration (fplbm.hpp:233-236)
 template <class T, int D>
-void integrate(Part<T>& p, const T* body_acc, uint64_t sub_steps) {
+void integrate(Part<T>& p, const T* body_acc, uint64_t sub_steps, bool keep_ang = false) {
   uint64_t steps = sub_steps_of(sub_steps);
   T ts = T(T(1) / static_cast<T>(steps));
   for (int i = 0; i < D; ++i) p.acc[i] = T(T(p.force[i] / p.mass) + body_acc[i]);
-  if (D == 2) p.ang_acc[0] = T(p.torque[0] / p.inertia);
+  if (D == 2 && !keep_ang) p.ang_acc[0] = T(p.torque[0] / p.inertia);
   const int R = (D == 2) ? 1 : 3;
   for (uint64_t s = 0; s < steps; ++s) {
     T ts2 = T(ts * ts);
@@ -867,6 +871,35 @@ struct Sim final : SimBase {
         }
         break;
       }
+      case LBM_COLL_FPLBM: {  // fplbm.hpp:60-133 (rho, m as FpLbmReset computes them)
+        T r = T(0), m[3] = {T(0), T(0), T(0)}, u[3] = {T(0), T(0), T(0)}, eq[27];
+        for (int j = 0; j < Q; ++j) {  // compute_rho_momentum (macroscopic.hpp:41-60)
+          r = T(r + f[j]);
+          for (int d = 0; d < D; ++d) m[d] = T(m[d] + T(T(L::c[j][d]) * f[j]));
+        }
+        for (int d = 0; d < D; ++d) u[d] = T(m[d] / r);
+        rho[c] = r;
+        for (int d = 0; d < D; ++d) vel[c * D + d] = m[d];
+        equilibrium<T, L>(r, u, eq);
+        const T ics2 = T(3.0);
+        const T relax = T(T(1) - T(T(0.5f) * omega));
+        for (int i = 0; i < Q; ++i) {
+          T ci[3];
+          for (int d = 0; d < D; ++d) ci[d] = T(L::c[i][d]);
+          T cu = T(0);
+          for (int d = 0; d < D; ++d) cu = T(cu + T(ci[d] * u[d]));
+          T s2 = T(T(cu * ics2) * ics2);
+          T proj = T(0);
+          for (int d = 0; d < D; ++d) {
+            T t1 = T(T(ci[d] - u[d]) * ics2);
+            T t2 = T(ci[d] * s2);
+            proj = T(proj + T(T(t1 + t2) * force[c * D + d]));
+          }
+          T Fi = T(T(L::w[i]) * proj);
+          f[i] = T(T(f[i] + T(omega * T(eq[i] - f[i]))) + T(Fi * relax));
+        }
+        break;
+      }
       case LBM_COLL_HLBM: {  // hlbm.hpp:62-105
         T r, u[3], eq[27];
         rho_u<T, L>(f, r, u);
@@ -1027,6 +1060,54 @@ struct Sim final : SimBase {
     ABS[p] = abs64;
   }
 
+  // FpLbmOneParticle (fplbm.hpp:135-247) for particle p (its own AABB and
+  // radius): porosity assigned, cell force (u_s - m)(1 - B) with m the
+  // momentum of a flag-2 cell (FpLbmReset) or the stored macro velocity of
+  // any other cell, body force the negative sum.  Integration follows.
+  void gather_fplbm(uint32_t p, const T* a) {
+    Part<T>& P = parts[p];
+    const T one = T(1.0f), k = T(1.0f), eps = T(1.5f);
+    const T sss = static_cast<T>(pp.sub_steps);
+    T vs[3];
+    for (int d = 0; d < D; ++d) vs[d] = T(T(P.pos[d] - P.pos_old[d]) * sss);
+    uint64_t lo[3] = {0, 0, 0}, hi[3] = {1, 1, 1};
+    aabb<T, D>(P.pos, P.radius, n, lo, hi);
+    T fp[3] = {T(0), T(0), T(0)};
+    double f64[3] = {0, 0, 0}, abs64 = 0;
+    uint64_t x[3] = {0, 0, 0};
+    for (x[0] = lo[0]; x[0] < hi[0]; ++x[0])
+      for (x[1] = lo[1]; x[1] < hi[1]; ++x[1])
+        for (x[2] = (D == 3 ? lo[2] : 0); x[2] < (D == 3 ? hi[2] : 1); ++x[2]) {
+          const uint64_t c = cell(x);
+          T rel[3];
+          for (int d = 0; d < D; ++d) rel[d] = T(static_cast<T>(x[d]) - P.pos[d]);
+          const T B = porosity<T, D>(rel, P.radius, eps);
+          por[c] = B;
+          const T flip = T(one - B);
+          T mom[3] = {T(0), T(0), T(0)};
+          if (info[c] == 2) {
+            const T* f = a + gcell(x) * Q;
+            for (int j = 0; j < Q; ++j)
+              for (int d = 0; d < D; ++d) mom[d] = T(mom[d] + T(T(L::c[j][d]) * f[j]));
+          } else {
+            for (int d = 0; d < D; ++d) mom[d] = vel[c * D + d];
+          }
+          for (int d = 0; d < D; ++d) {
+            const T fc = T(T(T(vs[d] - mom[d]) * k) * flip);
+            force[c * D + d] = fc;
+            fp[d] = T(fp[d] - fc);
+            f64[d] -= static_cast<double>(fc);
+            abs64 += std::fabs(static_cast<double>(fc));
+          }
+        }
+    for (int d = 0; d < D; ++d) P.force[d] = fp[d];
+    for (int d = 0; d < 3; ++d) {
+      F64[p * 3 + d] = f64[d];
+      T64[p * 3 + d] = 0.0;
+    }
+    ABS[p] = abs64;
+  }
+
   // HlbmOneParticleStamp (hlbm.hpp:264-308): after the move, every cell of
   // the post-move AABB gains D += 1 - porosity and N += u_p (1 - porosity).
   void stamp_hlbm() {
@@ -1101,6 +1182,19 @@ struct Sim final : SimBase {
   // in index order, then (product mode) the cell_porosity composition.
   void couple(uint64_t t) {
     const T* a = A(t);
+    if (pp.coupling == LBM_COUPLING_FPLBM) {
+      // FpLbmReset (fplbm.hpp:21-58): porosity 1, force 0; the momentum of
+      // flag-2 cells is taken from the populations where the gather needs it
+      std::fill(por.begin(), por.end(), T(1.0));
+      std::fill(force.begin(), force.end(), T(0.0));
+      for (auto& q : parts)
+        for (int d = 0; d < 3; ++d) {
+          q.force[d] = T(0);
+          q.torque[d] = T(0);
+        }
+      for (uint32_t p = 0; p < parts.size(); ++p) gather_fplbm(p, a);
+      return;
+    }
     if (pp.coupling == LBM_COUPLING_HLBM) {
       // HlbmReset (hlbm.hpp:28-49), ParticleForceReset, gathers in index order
       std::fill(por.begin(), por.end(), T(1.0));
@@ -1156,8 +1250,11 @@ struct Sim final : SimBase {
     }
   }
   void integrate_all() override {
+    // FpLbmOneParticle integrates only when sub_steps > 0 (fplbm.hpp:238-244)
+    if (pp.coupling == LBM_COUPLING_FPLBM && pp.sub_steps == 0) return;
     T g[3] = {T(pp.body_acceleration[0]), T(pp.body_acceleration[1]), T(pp.body_acceleration[2])};
-    for (auto& p : parts) integrate<T, D>(p, g, pp.sub_steps);
+    const bool fp = pp.coupling == LBM_COUPLING_FPLBM;
+    for (auto& p : parts) integrate<T, D>(p, g, pp.sub_steps, fp);
   }
   uint64_t overflow() const override { return cl.overflow; }
   void get_hlbm(double* nsum, double* dsum) const override {

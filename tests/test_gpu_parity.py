@@ -487,3 +487,63 @@ def test_hlbm_particle_laden_run(hip, orc, st):
     assert np.max(np.abs(rg - ro)) <= ftol
     assert np.max(np.abs(ug - uo)) <= ftol * np.max(np.abs(uo))
     assert np.max(np.abs(g.get_porosity() - o.get_porosity())) <= (1e-13 if st == abi.F64 else 1e-6)
+
+
+@pytest.mark.parametrize("st", [abi.F64, abi.F32])
+def test_fplbm_collision_on_force_field_bitwise(hip, orc, st):
+    """The FPLBM collision (fplbm.hpp:60-133) with a given cell force field and
+    no particles: bitwise against the oracle, both AA phases; the macros hold
+    rho and the momentum."""
+    ext = (40, 18)
+    g, o = _pair(hip, orc, abi.D2Q9, st, ext)
+    flags = channel_flags_2d(*ext)
+    ops = {1: abi.op_desc(boundary=abi.BC_BOUNCE_BACK), 2: abi.op_desc(collision=abi.COLL_FPLBM, tau=0.8),
+           3: abi.op_desc(boundary=abi.BC_ZOUHE_PRESSURE_X, collision=abi.COLL_BGK, east=1, tau=0.8,
+                          ramp=(1.0, 0.01 / 8, 1.01, 8)),
+           4: abi.op_desc(boundary=abi.BC_ZOUHE_PRESSURE_X, collision=abi.COLL_BGK, east=0, tau=0.8, rho=1.0)}
+    rng = np.random.default_rng(43)
+    force = rng.uniform(-1e-4, 1e-4, 2 * ext[0] * ext[1])
+    _setup((g, o), flags, ops, force=force)
+    for n in (1, 2, 37):
+        g.step(n)
+        o.step(n)
+        _assert_same(g, o, exact=True)
+
+
+@pytest.mark.parametrize("st", [abi.F64, abi.F32])
+def test_fplbm_particle_laden_run(hip, orc, st):
+    """FPLBM coupling (FpLbmReset, FpLbmOneParticle, FPLBM collision; driver
+    examples/moving_poiseulle_particles_2d_fplbm_gpu/sim.cpp:185-258) with 6
+    discs for 60 steps against the oracle."""
+    ext = (96, 48)
+    g, o = _pair(hip, orc, abi.D2Q9, st, ext)
+    flags = channel_flags_2d(*ext)
+    ops = {1: abi.op_desc(boundary=abi.BC_BOUNCE_BACK), 2: abi.op_desc(collision=abi.COLL_FPLBM, tau=0.8),
+           3: abi.op_desc(boundary=abi.BC_ZOUHE_PRESSURE_X, collision=abi.COLL_BGK, east=1, tau=0.8,
+                          ramp=(1.0, 0.01 / 8, 1.01, 8)),
+           4: abi.op_desc(boundary=abi.BC_ZOUHE_PRESSURE_X, collision=abi.COLL_BGK, east=0, tau=0.8, rho=1.0)}
+    _setup((g, o), flags, ops)
+    discs = seeded_discs(6, ext, 4.0, 7.0, 19, margin=4.0, gap=4.0)
+    parts = [make_particle([x, y], r, 1.5, 2, st, orc=orc) for (x, y, r) in discs]
+    prm = abi.particle_params(k_n=0.0, zeta_n=0.0, walls=False, sub_steps=16,
+                              body_acceleration=(0.0, -1e-4), max_radius=7.0, coupling=abi.COUPLING_FPLBM)
+    for s in (g, o):
+        s.particles_set(parts, prm)
+    g.step(60)
+    o.step(60)
+    pg, po = g.particles_get(), o.particles_get()
+    moved = max(abs(a.position[d] - (x, y)[d]) for a, (x, y, _) in zip(po, discs) for d in range(2))
+    assert moved > 1e-4
+    tol = 1e-11 if st == abi.F64 else 1e-4
+    for a, b in zip(pg, po):
+        for d in range(2):
+            assert abs(a.position[d] - b.position[d]) <= tol * max(abs(b.position[d]), 1.0)
+    rg, ug = g.get_macros()
+    ro, uo = o.get_macros()
+    ftol = 1e-12 if st == abi.F64 else 1e-4
+    assert np.max(np.abs(rg - ro)) <= ftol
+    assert np.max(np.abs(ug - uo)) <= ftol * np.max(np.abs(uo))
+    assert np.max(np.abs(g.get_porosity() - o.get_porosity())) <= (1e-13 if st == abi.F64 else 1e-6)
+    ff_tol = 1e-12 if st == abi.F64 else 1e-4
+    fg, fo = g.get_force_field(), o.get_force_field()
+    assert np.max(np.abs(fg - fo)) <= ff_tol * max(np.max(np.abs(fo)), 1e-12)

@@ -759,3 +759,77 @@ def test_hlbm_stamp_after_move(orc):
     assert np.array_equal(dsum, want_d)
     assert np.allclose(nsum, up[:, None, None] * want_d[None], rtol=1e-15, atol=0)
     assert dsum.max() > 0.9
+
+
+# ------------------------------------------------------------------ FPLBM
+@pytest.mark.parametrize("lat", [abi.D2Q9, abi.D3Q19, abi.D3Q27])
+def test_fplbm_collision_is_guo_with_cell_force(orc, lat):
+    """fplbm.hpp:60-133: Guo forcing with the cell force field and u = m/rho,
+    no velocity shift -- the BGKGuo arithmetic with F = the cell force, bit
+    for bit; the macros it leaves are rho and the momentum m (FpLbmReset)."""
+    Dd, Q, c, w, opp = orc.lattice(lat)
+    rng = np.random.default_rng(51)
+    for _ in range(5):
+        f = _rand_f(lat, rng)
+        F = rng.uniform(-1e-3, 1e-3, Dd)
+        guo, rg, ug, _ = orc.cell_op(lat, F64, abi.op_desc(collision=abi.COLL_BGK_GUO, tau=0.7,
+                                                           flags=abi.OPF_GUO_CELL_FORCE), f, cell_force=F)
+        fp, rp, mp, _ = orc.cell_op(lat, F64, abi.op_desc(collision=abi.COLL_FPLBM, tau=0.7), f, cell_force=F)
+        assert (fp == guo).all() and rp == rg
+        assert np.allclose(mp, c[:, :Dd].T @ f, rtol=1e-14, atol=1e-17)
+        assert np.allclose(mp / rp, ug, rtol=1e-14, atol=1e-17)
+
+
+def test_fplbm_particle_brute_force(orc):
+    """FpLbmOneParticle (fplbm.hpp:135-247) against a direct numpy evaluation:
+    porosity assigned over the AABB, cell force (u_s - m)(1 - B) with u_s =
+    (X - X_old) * sub_steps and m the momentum of each flag-2 cell, body force
+    the negative sum; the force field is zero outside the AABB."""
+    lat = abi.D2Q9
+    D, Q, c, w, opp = orc.lattice(lat)
+    ext = (26, 20)
+    N = ext[0] * ext[1]
+    rng = np.random.default_rng(52)
+    s = orc.Sim(lat, F64, ext)
+    s.set_flags(np.full(N, 2, dtype=np.uint8))
+    f = np.array([orc.equilibrium(lat, F64, 1.0, [0.02, -0.01]) for _ in range(N)]).T
+    f = f * (1 + 0.05 * rng.uniform(-1, 1, size=f.shape))
+    s.set_populations(f)
+    s.set_force_field(np.full(2 * N, 7.0))  # FpLbmReset clears it
+    pos, R, ss = [12.4, 9.3], 4.5, 4
+    p = make_particle(pos, R, 1.5, 2, F64, vel=[0.01 / ss, -0.004 / ss], orc=orc)
+    s.particles_set([p], abi.particle_params(sub_steps=ss, max_radius=R, coupling=abi.COUPLING_FPLBM))
+    s.psm_couple()
+    q = s.particles_get()[0]
+    fs = f.reshape(Q, ext[1], ext[0])
+    us = np.array([(pos[d] - p.position_old[d]) * ss for d in range(2)])
+    a = [int(max(pos[d] - R, 0)) for d in range(2)]
+    b = [min(int(pos[d] + R + 1), ext[d]) for d in range(2)]
+    Fp = np.zeros(2)
+    cf_ref = np.zeros((2, ext[1], ext[0]))
+    por_ref = np.ones((ext[1], ext[0]))
+    for x in range(a[0], b[0]):
+        for y in range(a[1], b[1]):
+            B = orc.porosity(2, F64, [x - pos[0], y - pos[1]], R, 1.5)
+            por_ref[y, x] = B
+            m = c[:, :2].T @ fs[:, y, x]
+            cf_ref[:, y, x] = (us - m) * (1 - B)
+            Fp -= cf_ref[:, y, x]
+    assert np.allclose([q.force[0], q.force[1]], Fp, rtol=1e-12, atol=1e-15)
+    assert (s.get_porosity().reshape(ext[1], ext[0]) == por_ref).all()
+    assert np.allclose(s.get_force_field().reshape(2, ext[1], ext[0]), cf_ref, rtol=1e-13, atol=1e-17)
+
+
+def test_fplbm_zero_sub_steps_does_not_move(orc):
+    """FpLbmOneParticle integrates only for sub_steps > 0 (fplbm.hpp:238-244)."""
+    ext = (20, 16)
+    s = orc.Sim(abi.D2Q9, F64, ext)
+    s.set_flags(np.full(ext[0] * ext[1], 2, dtype=np.uint8))
+    s.set_operator(2, abi.op_desc(collision=abi.COLL_FPLBM, tau=0.8))
+    s.init_equilibrium(1.0, (0.01, 0.0))
+    p = make_particle([9.5, 8.2], 3.0, 1.5, 2, F64, vel=[0.001, 0.0], orc=orc)
+    s.particles_set([p], abi.particle_params(sub_steps=0, max_radius=3.0, coupling=abi.COUPLING_FPLBM,
+                                             body_acceleration=(0.0, -1e-3)))
+    s.step(3)
+    q = s.particles_get()[0]
+    assert list(q.position)[:2] == list(p.position)[:2]
