@@ -122,7 +122,6 @@ def bench_c3(args, storage):
     steps = args.steps if args.steps != 1000 else 2000
 
     hlbm = args.coupling == "hlbm"
-    fplbm = args.coupling == "fplbm"
     fluid = {"psm": abi.COLL_PSM, "hlbm": abi.COLL_HLBM, "fplbm": abi.COLL_FPLBM}[args.coupling]
     edge = abi.COLL_HLBM if hlbm else abi.COLL_BGK
     coupling = {"psm": abi.COUPLING_PSM, "hlbm": abi.COUPLING_HLBM, "fplbm": abi.COUPLING_FPLBM}[args.coupling]
@@ -355,6 +354,89 @@ def bench_c5(args):
                        "final_loss": float(losses[-50:].mean()), "wall_s": round(time.perf_counter() - t_all, 1)},
             "roofline": None}
 
+def bench_writers(args):
+    """Result output on the headline lattice (D3Q19 FP32 512x128x128, the
+    channel after 20 steps): legacy VTK and CSV of the drivers' MacroStruct
+    (velocity, density) and a Forst checkpoint (info, dfs, velocity,
+    density) written and read back, each timed end to end (device read-back,
+    formatting on the host cores, file I/O).  CPU baseline: the reference's
+    own write_vtk_file, built from its sources (oracle/_ref), on a 128x64x32
+    float struct of the same members."""
+    import ctypes as C
+    import shutil
+    import tempfile
+    import numpy as np
+    from lbm_mirror_amd import abi, output
+    from lbm_mirror_amd.lattice import Lattice
+    from helpers import duct_flags_3d, c2_ops
+    ext = (512, 128, 128)
+    lat = Lattice(abi.D3Q19, abi.F32, ext)
+    lat.set_flags(duct_flags_3d(*ext))
+    lat.set_periodic(1)
+    for fl, op in c2_ops(0.6, 1e-6).items():
+        lat.set_operator(fl, op)
+    lat.init_equilibrium(1.0, (0.0, 0.0, 0.0))
+    lat.step(20)
+    lat.sync()
+    N = lat.N
+    d = tempfile.mkdtemp(prefix="lbm_out_")
+    res = {}
+    try:
+        def timed(name, fn, paths):
+            t0 = time.perf_counter()
+            fn()
+            dt = time.perf_counter() - t0
+            size = sum(os.path.getsize(p) for p in paths)
+            res[name] = {"s": round(dt, 3), "MB": round(size / 1e6, 1),
+                         "Mcells_per_s": round(N / dt / 1e6, 2), "MB_per_s": round(size / dt / 1e6, 1)}
+            print(f"[bench] {name}: {dt:.3f} s, {size / 1e6:.1f} MB", file=sys.stderr, flush=True)
+
+        vtk = os.path.join(d, "vtk", "m_20.vtk")
+        timed("vtk", lambda: output.write_vtk(lat, d, "m", 20), [vtk])
+        csvs = [os.path.join(d, "csv", f"m_{k}_20.csv") for k in ("velocity", "density")]
+        timed("csv", lambda: output.write_csv(lat, d, "m", 20), csvs)
+        ck = os.path.join(d, "ckpt.forst")
+        members = "info,dfs,velocity,density"
+        timed("forst_write", lambda: output.write_forst(lat, ck, members), [ck])
+        timed("forst_read", lambda: output.read_forst(lat, ck, members), [ck])
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    lat.close()
+    out = {"metric": "VTK write rate (velocity+density, D3Q19 FP32 512x128x128, incl. device read-back)",
+           "value": res["vtk"]["Mcells_per_s"], "unit": "Mcells/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+           "ms_per_step": round(res["vtk"]["s"] * 1e3, 1), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "result output of the 512x128x128 D3Q19 channel after 20 steps",
+                      "host_threads": min(16, os.cpu_count() or 1), "formats": res},
+           "roofline": None}
+    ref = os.path.join(ROOT, "oracle", "_ref", "libref_harness.so")
+    if not args.no_cpu and os.path.exists(ref):
+        L = C.CDLL(ref)
+        L.ref_write_fields.restype = C.c_int64
+        dp = C.POINTER(C.c_double)
+        n = 128 * 64 * 32
+        rng = np.random.default_rng(1)
+        vel = 0.01 * rng.standard_normal(3 * n)
+        rho = 1.0 + 1e-3 * rng.standard_normal(n)
+        zero = np.zeros(19 * n)
+        info = np.ones(n, np.uint8)
+        sec = C.c_double()
+        rd = tempfile.mkdtemp(prefix="lbm_ref_out_")
+        try:
+            r = L.ref_write_fields(2, abi.F32, 0, vel.ctypes.data_as(dp), rho.ctypes.data_as(dp),
+                                   zero.ctypes.data_as(dp), zero.ctypes.data_as(dp),
+                                   info.ctypes.data_as(C.POINTER(C.c_uint8)), zero.ctypes.data_as(dp),
+                                   rd.encode(), b"m", C.c_uint64(20), 0, None, C.c_uint64(0), C.byref(sec))
+        finally:
+            shutil.rmtree(rd, ignore_errors=True)
+        if r == 0:
+            out["cpu_baseline"] = {"value": round(n / sec.value / 1e6, 3), "unit": "Mcells/s", "cores": 1,
+                                   "kind": "reference",
+                                   "sample": "reference write_vtk_file (write_vtk.hpp, built from its sources) "
+                                             "of a float velocity+density struct, 128x64x32 cells, host arrays"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -364,10 +446,10 @@ def main():
     ap.add_argument("--storage", default="F32", choices=["F32", "F64", "F16S", "BF16S"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
-    ap.add_argument("--config", default="c2", choices=["c2", "c2b", "c1", "c3", "c4", "c5"],
+    ap.add_argument("--config", default="c2", choices=["c2", "c2b", "c1", "c3", "c4", "c5", "writers"],
                     help="c2: the headline D3Q19 channel (default); c2b: D3Q27 256^3; "
                          "c1: D2Q9 FP64 channel; c3: particle-laden; c4: ragged batch; "
-                         "c5: learned warm start")
+                         "c5: learned warm start; writers: VTK/CSV/Forst output")
     ap.add_argument("--train-samples", type=int, default=1024)
     ap.add_argument("--test-samples", type=int, default=64)
     ap.add_argument("--train-steps", type=int, default=3000)
@@ -393,6 +475,9 @@ def main():
         args.lattice = "D3Q27"
     if args.config == "c5":
         print(json.dumps(bench_c5(args)), flush=True)
+        return
+    if args.config == "writers":
+        print(json.dumps(bench_writers(args)), flush=True)
         return
     if args.config in ("c1", "c3", "c4"):
         st = {"F32": abi.F32, "F16S": abi.F16S, "BF16S": abi.BF16S, "F64": abi.F32}[args.storage]

@@ -164,6 +164,10 @@ lbm_status lbm_ctx_destroy(lbm_ctx* ctx);
  * LBM_ERR_NOT_SUPPORTED: the single-buffer AA layout cannot hold the
  * reference's stale second buffer for such cells. */
 lbm_status lbm_set_flags(lbm_ctx* ctx, const uint8_t* flags);
+lbm_status lbm_get_flags(lbm_ctx* ctx, uint8_t* flags);
+/* Lattice, storage and extent the context was created with (the
+ * compile-time Desc, T and chunk Sides of the reference's field types). */
+lbm_status lbm_ctx_info(lbm_ctx* ctx, int32_t* lattice, int32_t* storage, uint64_t extent[3]);
 /* Periodic wrap per axis (bit 0 = x, 1 = y, 2 = z).  No reference
  * counterpart: the reference only has zero ghosts (stream.hpp:47). */
 lbm_status lbm_set_periodic(lbm_ctx* ctx, uint32_t axis_mask);
@@ -205,6 +209,67 @@ lbm_status lbm_sync(lbm_ctx* ctx);
 /* Device time of the last lbm_step call, from HIP events recorded on the
  * context's stream around its launches. */
 lbm_status lbm_last_step_ms(lbm_ctx* ctx, double* ms);
+
+/* ---- result output: write_vtk.hpp, write_csv.hpp, abstract/forst.hpp ------
+ * `members` lists the fields of the struct being written, comma-separated, in
+ * file order; each is one of the reference drivers' member keys
+ *   velocity, force   Vector<T,D> chunks (lbm_get_macros / lbm_get_force_field)
+ *   density, porosity Scalar<T> chunks
+ *   info              the UInt8 flag chunk
+ *   dfs               the FixedArray<T,Q> population chunk (ghost ring 1), holding
+ *                     the populations of lbm_get_populations
+ * (e.g. the MacroStruct of examples/stokes_drag_particle_2d_hlbm_gpu/sim.cpp:
+ * 47-52 is "velocity,density,force,porosity").  T is double for F64 lattices
+ * and float otherwise.  Values are printed as the reference's ostream does
+ * (%g, 6 significant digits), so the files are byte-identical to the
+ * reference writers' for the same field values. */
+
+/* Host copies of a lattice's fields, for writing results that are already on
+ * the host (batch members, post-processing) and for reading snapshots.  SoA
+ * doubles, cell x-fastest, like the get/set calls; a member that is listed
+ * needs its buffer, the others may be NULL. */
+typedef struct lbm_field_set {
+  int32_t lattice;   /* lbm_lattice: D and Q                                 */
+  int32_t storage;   /* lbm_storage: T = double for F64, float otherwise       */
+  uint64_t extent[3];
+  double* velocity;  /* D*N */
+  double* density;   /* N   */
+  double* force;     /* D*N */
+  double* porosity;  /* N   */
+  uint8_t* info;     /* N   */
+  double* dfs;       /* Q*N */
+} lbm_field_set;
+
+/* write_vtk_file (write_vtk.hpp:218-252): <out_dir>/vtk/<file_name>_<t>.vtk,
+ * legacy ASCII STRUCTURED_POINTS; directories are created. */
+lbm_status lbm_write_vtk(lbm_ctx* ctx, const char* out_dir, const char* file_name, uint64_t t,
+                         const char* members);
+/* write_csv_file (write_csv.hpp:168-181): one file per member,
+ * <out_dir>/csv/<file_name>_<member>_<t>.csv, one cell per line (x fastest),
+ * components comma-separated. */
+lbm_status lbm_write_csv(lbm_ctx* ctx, const char* out_dir, const char* file_name, uint64_t t,
+                         const char* members);
+/* codec<Struct<Member<Chunk...>...>, enc::Forst>::encode (abstract/forst.hpp:
+ * 580-600, forst_chunk.hpp:29-45) written to `path`: the members' chunk bytes
+ * back to back, little endian, no framing; a chunk covers its ghost ring
+ * (zeros here).  Populations need F64 or F32 storage. */
+lbm_status lbm_write_forst(lbm_ctx* ctx, const char* path, const char* members);
+/* codec<...>::decode (abstract/forst.hpp:602-609) from `path` into the
+ * context: info -> lbm_set_flags, dfs -> lbm_set_populations (ghost ring
+ * ignored), velocity/density -> lbm_set_macros, force, porosity.  A file
+ * shorter than the struct is rejected (err::forst_truncated) with
+ * LBM_ERR_INVALID_ARG; bytes past it are ignored, as the reference does.
+ * With "info,dfs,velocity,density" (plus force, porosity when used) a
+ * context restored from lbm_write_forst continues bit for bit. */
+lbm_status lbm_read_forst(lbm_ctx* ctx, const char* path, const char* members);
+/* The same four on host buffers; lbm_read_forst_fields fills the listed
+ * members' buffers of `set` (extent, lattice and storage say what to expect). */
+lbm_status lbm_write_vtk_fields(const lbm_field_set* set, const char* out_dir, const char* file_name,
+                                uint64_t t, const char* members);
+lbm_status lbm_write_csv_fields(const lbm_field_set* set, const char* out_dir, const char* file_name,
+                                uint64_t t, const char* members);
+lbm_status lbm_write_forst_fields(const lbm_field_set* set, const char* path, const char* members);
+lbm_status lbm_read_forst_fields(lbm_field_set* set, const char* path, const char* members);
 
 /* ---- particles ------------------------------------------------------------ */
 lbm_status lbm_particles_set(lbm_ctx* ctx, const lbm_particle* p, uint32_t n,

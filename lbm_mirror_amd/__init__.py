@@ -41,6 +41,9 @@ EXPORTS = [
     "lbm_net_clear_samples", "lbm_net_sample_count", "lbm_net_train", "lbm_net_predict",
     "lbm_net_forward_raw", "lbm_net_loss_grad_raw", "lbm_net_last_ms",
     "lbm_net_get_output_scale", "lbm_net_set_output_scale",
+    "lbm_get_flags", "lbm_ctx_info", "lbm_write_vtk", "lbm_write_csv", "lbm_write_forst",
+    "lbm_read_forst", "lbm_write_vtk_fields", "lbm_write_csv_fields", "lbm_write_forst_fields",
+    "lbm_read_forst_fields",
 ]
 
 
@@ -114,6 +117,18 @@ def load_library(path=None):
     L.lbm_net_last_ms.argtypes = [vp, dp]
     L.lbm_net_get_output_scale.argtypes = [vp, fp, fp]
     L.lbm_net_set_output_scale.argtypes = [vp, fp, fp]
+    L.lbm_get_flags.argtypes = [vp, u8p]
+    L.lbm_ctx_info.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), u64p]
+    cs = C.c_char_p
+    L.lbm_write_vtk.argtypes = [vp, cs, cs, C.c_uint64, cs]
+    L.lbm_write_csv.argtypes = [vp, cs, cs, C.c_uint64, cs]
+    L.lbm_write_forst.argtypes = [vp, cs, cs]
+    L.lbm_read_forst.argtypes = [vp, cs, cs]
+    fsp = C.POINTER(abi.FieldSet)
+    L.lbm_write_vtk_fields.argtypes = [fsp, cs, cs, C.c_uint64, cs]
+    L.lbm_write_csv_fields.argtypes = [fsp, cs, cs, C.c_uint64, cs]
+    L.lbm_write_forst_fields.argtypes = [fsp, cs, cs]
+    L.lbm_read_forst_fields.argtypes = [fsp, cs, cs]
     for name in EXPORTS:
         f = getattr(L, name)
         if name not in ("lbm_last_error", "lbm_version", "lbm_device_count", "lbm_batch_member"):

@@ -134,6 +134,21 @@ def particle_params(k_n=1.0, zeta_n=0.3, gamma_t=0.0, mu=0.0, walls=False,
     return p
 
 
+class FieldSet(C.Structure):
+    """lbm_field_set: host copies of a lattice's fields (SoA doubles)."""
+    _fields_ = [
+        ("lattice", C.c_int32),
+        ("storage", C.c_int32),
+        ("extent", C.c_uint64 * 3),
+        ("velocity", C.POINTER(C.c_double)),
+        ("density", C.POINTER(C.c_double)),
+        ("force", C.POINTER(C.c_double)),
+        ("porosity", C.POINTER(C.c_double)),
+        ("info", C.POINTER(C.c_uint8)),
+        ("dfs", C.POINTER(C.c_double)),
+    ]
+
+
 class BatchTuning(C.Structure):
     _fields_ = [
         ("block", C.c_int32),

@@ -373,6 +373,23 @@ lbm_status lbm_set_flags(lbm_ctx* c, const uint8_t* flags) {
   return LBM_OK;
 }
 
+lbm_status lbm_get_flags(lbm_ctx* c, uint8_t* flags) {
+  if (!c || !flags) return fail(LBM_ERR_INVALID_ARG, "null argument");
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipMemcpyAsync(flags, c->flags, c->N, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return LBM_OK;
+}
+
+lbm_status lbm_ctx_info(lbm_ctx* c, int32_t* lattice, int32_t* storage, uint64_t extent[3]) {
+  if (!c) return fail(LBM_ERR_INVALID_ARG, "null ctx");
+  if (lattice) *lattice = c->lattice;
+  if (storage) *storage = c->storage;
+  if (extent)
+    for (int d = 0; d < 3; ++d) extent[d] = c->n[d];
+  return LBM_OK;
+}
+
 lbm_status lbm_set_periodic(lbm_ctx* c, uint32_t mask) {
   if (!c) return fail(LBM_ERR_INVALID_ARG, "null ctx");
   if (mask > 7) return fail(LBM_ERR_INVALID_ARG, "axis mask %u", mask);

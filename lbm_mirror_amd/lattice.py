@@ -53,6 +53,11 @@ class Lattice:
             raise ValueError(f"flags: expected {self.N} cells, got {f.size}")
         check(self.L.lbm_set_flags(self._h, f.ctypes.data_as(C.POINTER(C.c_uint8))))
 
+    def get_flags(self):
+        f = np.zeros(self.N, dtype=np.uint8)
+        check(self.L.lbm_get_flags(self._h, f.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return f
+
     def set_periodic(self, mask):
         check(self.L.lbm_set_periodic(self._h, mask))
 

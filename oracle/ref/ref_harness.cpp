@@ -15,12 +15,16 @@
 //   reset_particle_accumulators / integrate_particle  particle/integrate.hpp
 //   make_cell_list / build_cell_list / cell_porosity  particle/cell_list.hpp,
 //                                                       particle/couple.hpp
+//   write_vtk_file / write_csv_file   write_vtk.hpp, write_csv.hpp
+//   codec<Struct<Chunk...>, enc::Forst>::encode   abstract/forst.hpp, forst_chunk.hpp
 // The fluid components (BGK, BGKGuo, Stream, Zou/He, Psm, the PSM gather)
 // take Chunk field views that only the SYCL backend provides
 // (modules/sycl/c++/data.hpp:323-396); AdaptiveCpp is absent, so they are not
 // built here.
+#include <chrono>
 #include <cstdint>
 #include <cstring>
+#include <memory>
 
 #include "modules/core/c++/equilibrium.hpp"
 #include "modules/core/c++/macroscopic.hpp"
@@ -29,6 +33,9 @@
 #include "modules/core/c++/particle/integrate.hpp"
 #include "modules/core/c++/particle/particle.hpp"
 #include "modules/core/c++/particle/porosity.hpp"
+#include "modules/core/c++/forst_chunk.hpp"
+#include "modules/core/c++/write_csv.hpp"
+#include "modules/core/c++/write_vtk.hpp"
 
 #include "../../include/lbm_mirror_amd/lbm.h"
 
@@ -199,6 +206,89 @@ uint64_t couple_t(const lbm_particle* p, const uint64_t* extent, double max_radi
   return cl.template get<"overflow">().at({0u}).get();
 }
 
+
+// ---- result writers ---------------------------------------------------------
+// The drivers' field structs (e.g. examples/stokes_drag_particle_2d_hlbm_gpu/
+// sim.cpp:28-52): macros as ghost-0 chunks, populations as a ghost-1 chunk.
+template <typename T, uint64_t D, uint64_t Q, uint64_t... Dims>
+struct out_schema {
+  using Vel = sch::Chunk<sch::Vector<T, D>, 0u, Dims...>;
+  using Rho = sch::Chunk<sch::Scalar<T>, 0u, Dims...>;
+  using Info = sch::Chunk<sch::UInt8, 0u, Dims...>;
+  using Df = sch::Chunk<sch::FixedArray<T, Q>, 1u, Dims...>;
+  using Full = sch::Struct<sch::Member<Vel, "velocity">, sch::Member<Rho, "density">,
+                           sch::Member<Vel, "force">, sch::Member<Rho, "porosity">,
+                           sch::Member<Info, "info">, sch::Member<Df, "dfs">>;
+  using Macro = sch::Struct<sch::Member<Vel, "velocity">, sch::Member<Rho, "density">>;
+};
+
+template <uint64_t... Dims>
+kl::data<sch::FixedArray<sch::UInt64, sizeof...(Dims)>> cell_index(uint64_t c) {
+  constexpr uint64_t ext[] = {Dims...};
+  kl::data<sch::FixedArray<sch::UInt64, sizeof...(Dims)>> idx;
+  for (uint64_t d = 0; d < sizeof...(Dims); ++d) {
+    idx.at({d}).set(c % ext[d]);
+    c /= ext[d];
+  }
+  return idx;
+}
+
+struct out_in {
+  const double *vel, *rho, *force, *por, *dfs;
+  const uint8_t* info;
+};
+
+template <typename T, uint64_t D, uint64_t Q, bool Full, uint64_t... Dims, typename S>
+void fill_fields(kl::data<S>& f, const out_in& in) {
+  constexpr uint64_t N = (Dims * ...);
+  for (uint64_t c = 0; c < N; ++c) {
+    auto idx = cell_index<Dims...>(c);
+    for (uint64_t d = 0; d < D; ++d) f.template get<"velocity">().at(idx).at({{d}}).set(in.vel[d * N + c]);
+    f.template get<"density">().at(idx).at({}).set(in.rho[c]);
+    if constexpr (Full) {
+      for (uint64_t d = 0; d < D; ++d) f.template get<"force">().at(idx).at({{d}}).set(in.force[d * N + c]);
+      f.template get<"porosity">().at(idx).at({}).set(in.por[c]);
+      f.template get<"info">().at(idx).set(in.info[c]);
+      for (uint64_t i = 0; i < Q; ++i) f.template get<"dfs">().at(idx).at({i}).set(in.dfs[i * N + c]);
+    }
+  }
+}
+
+// what: 0 VTK, 1 CSV (files under out_dir as the reference names them),
+// 2 Forst encoding into buf; returns the byte count for 2, 0 otherwise, -1 on error.
+template <typename T, uint64_t D, uint64_t Q, uint64_t... Dims>
+int64_t write_t(int full, const out_in& in, const char* out_dir, const char* name, uint64_t t, int what,
+                uint8_t* buf, uint64_t cap) {
+  auto run = [&](auto* tag, auto full_c) -> int64_t {
+    using S = std::remove_pointer_t<decltype(tag)>;
+    auto f = std::make_unique<kl::data<S>>();
+    fill_fields<T, D, Q, decltype(full_c)::value, Dims...>(*f, in);
+    if (what == 0) return kl::write_vtk_file(std::filesystem::path{out_dir}, name, t, *f).is_value() ? 0 : -1;
+    if (what == 1) return kl::write_csv_file(std::filesystem::path{out_dir}, name, t, *f).is_value() ? 0 : -1;
+    kl::codec<S, kl::enc::Forst> codec;
+    kl::data<S, kl::enc::Forst> bytes;
+    if (!codec.encode(*f, bytes).is_value()) return -1;
+    const uint64_t n = bytes.size().get();
+    if (n > cap) return -1;
+    std::memcpy(buf, bytes.raw(), n);
+    return int64_t(n);
+  };
+  using OS = out_schema<T, D, Q, Dims...>;
+  return full ? run((typename OS::Full*)nullptr, std::true_type{})
+              : run((typename OS::Macro*)nullptr, std::false_type{});
+}
+
+template <typename T>
+int64_t write_dims(int shape, int full, const out_in& in, const char* dir, const char* name, uint64_t t, int what,
+                   uint8_t* buf, uint64_t cap) {
+  switch (shape) {
+    case 0: return write_t<T, 2, 9, 5u, 3u>(full, in, dir, name, t, what, buf, cap);
+    case 1: return write_t<T, 3, 19, 3u, 4u, 2u>(full, in, dir, name, t, what, buf, cap);
+    case 2: return write_t<T, 3, 19, 128u, 64u, 32u>(full, in, dir, name, t, what, buf, cap);
+  }
+  return -1;
+}
+
 }  // namespace
 
 extern "C" {
@@ -286,6 +376,22 @@ int64_t ref_couple(int st, uint32_t n, const lbm_particle* p, const uint64_t* ex
     return -1;
   };
   return st == LBM_F64 ? run(sch::Float64{}) : run(sch::Float32{});
+}
+
+// Result writers on the reference's own field structs.  shape 0: 2D 5x3
+// (velocity D=2, dfs Q=9); 1: 3D 3x4x2 (Q=19); 2: 3D 128x64x32 (timing).
+// full 1: {velocity, density, force, porosity, info, dfs}; 0: {velocity,
+// density}.  Inputs are SoA doubles (cell x-fastest), cast to the storage type.
+int64_t ref_write_fields(int shape, int st, int full, const double* vel, const double* rho,
+                         const double* force, const double* por, const uint8_t* info, const double* dfs,
+                         const char* out_dir, const char* name, uint64_t t, int what, uint8_t* buf,
+                         uint64_t cap, double* seconds) {
+  out_in in{vel, rho, force, por, dfs, info};
+  auto t0 = std::chrono::steady_clock::now();
+  int64_t r = st == LBM_F64 ? write_dims<sch::Float64>(shape, full, in, out_dir, name, t, what, buf, cap)
+                            : write_dims<sch::Float32>(shape, full, in, out_dir, name, t, what, buf, cap);
+  if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return r;
 }
 
 }  // extern "C"
