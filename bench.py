@@ -437,6 +437,85 @@ def bench_writers(args):
     return out
 
 
+def bench_sphere3d(args):
+    """The 3D HLBM drivers (examples/poiseulle_particle_3d_hlbm_gpu, and with
+    --spin examples/vawt_3d_hlbm_gpu): D3Q27 FP32, a sphere of radius 0.15*ny
+    at (0.3, 0.5, 0.5)*extent in a pressure-driven channel, HLBM coupling with
+    16 sub-steps; the free sphere has DEM wall contact, the rotor is pinned and
+    spins about +y (ParticleAxisIntegrate).  Timed at the drivers' own 32^3 and
+    at 256x128x128 with the sphere scaled.  The D3Q27 Zou/He faces use
+    LBM_OPF_ZOUHE_NORMAL_ONLY: the reference's reconstruction returns 4/3 of
+    the tangential momentum each step and these runs leave the finite numbers
+    after ~100 steps (--zouhe-reference times that formula instead)."""
+    import numpy as np
+    from lbm_mirror_amd import abi
+    from lbm_mirror_amd.lattice import Lattice, spheroid_mass
+    from lbm_mirror_amd.warmstart import init_equilibrium_field
+    from helpers import make_particle
+
+    def run(ext, steps):
+        nx, ny, nz = ext
+        lat = Lattice(abi.D3Q27, abi.F32, ext)
+        f = np.ones((nz, ny, nx), np.uint8)
+        f[1:-1, 1:-1, 1:-1] = 2
+        f[1:-1, 1:-1, 0] = 3
+        f[1:-1, 1:-1, nx - 1] = 4
+        lat.set_flags(f.reshape(-1))
+        tau = 0.8
+        zf = 0 if args.zouhe_reference else abi.OPF_ZOUHE_NORMAL_ONLY
+        ops = {1: abi.op_desc(boundary=abi.BC_BOUNCE_BACK),
+               2: abi.op_desc(collision=abi.COLL_HLBM, tau=tau),
+               3: abi.op_desc(boundary=abi.BC_ZOUHE_PRESSURE_X, collision=abi.COLL_HLBM, east=1, tau=tau,
+                              ramp=(1.0, 0.001 / 256, 1.001, 256), flags=zf),
+               4: abi.op_desc(boundary=abi.BC_ZOUHE_PRESSURE_X, collision=abi.COLL_HLBM, east=0, tau=tau,
+                              rho=1.0, flags=zf)}
+        for fl, op in ops.items():
+            lat.set_operator(fl, op)
+        interior = np.zeros((nz, ny, nx), bool)
+        interior[1:-1, 1:-1, 1:-1] = True
+        u = np.zeros((3, lat.N))
+        u[0, interior.reshape(-1)] = 0.01
+        init_equilibrium_field(lat, np.ones(lat.N), u)
+        r = 0.15 * ny
+        m, inertia = spheroid_mass(3, abi.F32, r, 1.0)
+        p = make_particle([0.3 * nx, 0.5 * ny, 0.5 * nz], r, 1.0, 3, abi.F32, mass=m, inertia=inertia)
+        if args.spin:
+            prm = abi.particle_params(k_n=0.0, zeta_n=0.0, walls=False, sub_steps=16, max_radius=r,
+                                      coupling=abi.COUPLING_HLBM, spin_axis=(0.0, 1.0, 0.0))
+        else:
+            prm = abi.particle_params(k_n=1.0e-2, zeta_n=0.3, walls=True, sub_steps=16, max_radius=r,
+                                      coupling=abi.COUPLING_HLBM)
+        lat.particles_set([p], prm)
+        lat.step(args.warmup, t0=0)
+        lat.sync()
+        lat.step(steps, t0=args.warmup)
+        lat.sync()
+        ms = lat.last_step_ms()
+        q = lat.particles_get()[0]
+        umax = float(np.max(np.abs(lat.get_macros()[1])))
+        lat.close()
+        cells = nx * ny * nz
+        return {"extent": list(ext), "steps": steps, "ms_per_step": round(ms / steps, 5),
+                "MLUPS": round(cells * steps / (ms * 1e3), 1),
+                "position": [round(x, 5) for x in q.position], "rotation_y": q.rotation[1],
+                "max_abs_u": umax}
+
+    small = run((32, 32, 32), 256)
+    big = run((256, 128, 128), args.steps if args.steps != 1000 else 400)
+    name = "rotor (ParticleAxisIntegrate)" if args.spin else "free sphere with wall contact"
+    return {"metric": f"MLUPS (D3Q27 FP32 HLBM {name}, full step incl. particle)",
+            "value": big["MLUPS"], "unit": "MLUPS", "n_gpus": 1, "steps": big["steps"],
+            "warmup": args.warmup, "ms_per_step": big["ms_per_step"], "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "3D HLBM sphere channel 256x128x128 (sphere r 19.2); the drivers' "
+                                   "own 32^3 (r 4.8) in 'driver_shape'",
+                       "zouhe": "reference formula" if args.zouhe_reference else "normal-only",
+                       "driver_shape": small, "scaled": big},
+            "roofline": {"bound": "hbm", "achieved": round(big["MLUPS"] * 1e6 * 216 / 1e9, 1),
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(big["MLUPS"] * 1e6 * 216 / 1e9 / HBM_PEAK_GBS, 4), "traffic": None}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -446,10 +525,11 @@ def main():
     ap.add_argument("--storage", default="F32", choices=["F32", "F64", "F16S", "BF16S"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
-    ap.add_argument("--config", default="c2", choices=["c2", "c2b", "c1", "c3", "c4", "c5", "writers"],
+    ap.add_argument("--config", default="c2", choices=["c2", "c2b", "c1", "c3", "c4", "c5", "writers", "sphere3d"],
                     help="c2: the headline D3Q19 channel (default); c2b: D3Q27 256^3; "
                          "c1: D2Q9 FP64 channel; c3: particle-laden; c4: ragged batch; "
-                         "c5: learned warm start; writers: VTK/CSV/Forst output")
+                         "c5: learned warm start; writers: VTK/CSV/Forst output; "
+                         "sphere3d: the 3D HLBM sphere / rotor drivers")
     ap.add_argument("--train-samples", type=int, default=1024)
     ap.add_argument("--test-samples", type=int, default=64)
     ap.add_argument("--train-steps", type=int, default=3000)
@@ -457,6 +537,9 @@ def main():
     ap.add_argument("--train-max-steps", type=int, default=100000)
     ap.add_argument("--test-max-steps", type=int, default=400000)
     ap.add_argument("--members", type=int, default=256)
+    ap.add_argument("--spin", action="store_true", help="sphere3d: the pinned rotor (vawt_3d_hlbm_gpu)")
+    ap.add_argument("--zouhe-reference", action="store_true",
+                    help="sphere3d: the reference's D3Q27 Zou/He formula (diverges after ~100 steps)")
     ap.add_argument("--coupling", default="psm", choices=["psm", "hlbm", "fplbm"],
                     help="particle coupling of --config c3")
     args = ap.parse_args()
@@ -478,6 +561,9 @@ def main():
         return
     if args.config == "writers":
         print(json.dumps(bench_writers(args)), flush=True)
+        return
+    if args.config == "sphere3d":
+        print(json.dumps(bench_sphere3d(args)), flush=True)
         return
     if args.config in ("c1", "c3", "c4"):
         st = {"F32": abi.F32, "F16S": abi.F16S, "BF16S": abi.BF16S, "F64": abi.F32}[args.storage]

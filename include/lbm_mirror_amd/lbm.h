@@ -93,6 +93,13 @@ typedef enum lbm_particle_coupling {
 #define LBM_OPF_PSM_STAMP_DISC  0x2u  /* cmpt::OneParticleAt before Psm (particle/particle_opa.hpp:14-46) */
 #define LBM_OPF_PSM_FORCE_GATHER 0x4u /* cmpt::ForceGather after Psm (momentum_gather.hpp:15-52)         */
 #define LBM_OPF_PSM_CELL_FORCE  0x8u  /* cmpt::PSM variant writing Σ(f_i-f_opp)c_i·B (component/psm.hpp:16-80) */
+/* D3Q27 Zou/He pressure without the tangential terms: each unknown is its
+ * mirror across the face plus 6 w_i ρu_x c_ix only.  No reference
+ * counterpart.  The reference's reconstruction (boundary.hpp:539-754) adds
+ * the tangential terms to mirror pairs, which leaves the cell with 4/3 of
+ * its tangential momentum every step; this variant keeps mass (Σf = ρ_set)
+ * and tangential momentum, so 3D channels stay stable.  Default off. */
+#define LBM_OPF_ZOUHE_NORMAL_ONLY 0x10u
 
 typedef struct lbm_op_desc {
   int32_t boundary;      /* lbm_boundary_kind                                   */
@@ -145,6 +152,11 @@ typedef struct lbm_particle_params {
                                        1: FP64 particle state and arithmetic whatever
                                        the storage (no reference counterpart)       */
   int32_t coupling;                 /* lbm_particle_coupling (HLBM needs scatter porosity) */
+  int32_t fixed_axis;               /* 0: ParticleIntegrate; 1: ParticleAxisIntegrate
+                                       (integrate.hpp:134-189,298-327): acceleration 0,
+                                       position pinned, spin about spin_axis only      */
+  double spin_axis[3];              /* fixed axis (normalised in the particle type as the
+                                       component does; 2D uses the sign of [0])        */
 } lbm_particle_params;
 
 typedef struct lbm_ctx lbm_ctx;

@@ -41,6 +41,7 @@ OPF_GUO_CELL_FORCE = 0x1
 OPF_PSM_STAMP_DISC = 0x2
 OPF_PSM_FORCE_GATHER = 0x4
 OPF_PSM_CELL_FORCE = 0x8
+OPF_ZOUHE_NORMAL_ONLY = 0x10
 
 POROSITY_SCATTER, POROSITY_PRODUCT = 0, 1
 
@@ -95,6 +96,8 @@ class ParticleParams(C.Structure):
         ("max_radius", C.c_double),
         ("state_fp64", C.c_int32),
         ("coupling", C.c_int32),
+        ("fixed_axis", C.c_int32),
+        ("spin_axis", C.c_double * 3),
     ]
 
 
@@ -121,7 +124,10 @@ def op_desc(boundary=BC_NONE, collision=COLL_NONE, east=0, flags=0, tau=1.0,
 def particle_params(k_n=1.0, zeta_n=0.3, gamma_t=0.0, mu=0.0, walls=False,
                     porosity_mode=POROSITY_SCATTER, psm_torque=False,
                     cell_capacity=4, sub_steps=1, body_acceleration=(0.0, 0.0, 0.0),
-                    eps=1.5, max_radius=1.0, state_fp64=False, coupling=COUPLING_PSM):
+                    eps=1.5, max_radius=1.0, state_fp64=False, coupling=COUPLING_PSM,
+                    spin_axis=None):
+    """spin_axis: None for ParticleIntegrate (free body), else the axis of
+    ParticleAxisIntegrate (pinned body spinning about it; 2D uses its sign)."""
     p = ParticleParams()
     p.k_n, p.zeta_n, p.gamma_t, p.mu = k_n, zeta_n, gamma_t, mu
     p.walls, p.porosity_mode, p.psm_torque = int(walls), porosity_mode, int(psm_torque)
@@ -131,6 +137,10 @@ def particle_params(k_n=1.0, zeta_n=0.3, gamma_t=0.0, mu=0.0, walls=False,
     p.eps, p.max_radius = eps, max_radius
     p.state_fp64 = int(state_fp64)
     p.coupling = coupling
+    if spin_axis is not None:
+        p.fixed_axis = 1
+        for i in range(3):
+            p.spin_axis[i] = spin_axis[i] if i < len(spin_axis) else 0.0
     return p
 
 

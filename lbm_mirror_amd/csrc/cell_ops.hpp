@@ -233,8 +233,10 @@ __device__ __forceinline__ void zouhe_p_d2q9(T* f, bool dir, T rs) {
 
 // ZouHePressureX<East>, D3Q27 (boundary.hpp:256-754); the body is written for
 // the x = max face and the flag is flipped once (AtXMax = !East, :248).
+// normal_only (LBM_OPF_ZOUHE_NORMAL_ONLY, no reference counterpart) drops the
+// tangential terms of the reconstruction.
 template <class T>
-__device__ __forceinline__ void zouhe_p_d3q27(T* f, bool east, T rs) {
+__device__ __forceinline__ void zouhe_p_d3q27(T* f, bool east, T rs, bool normal_only = false) {
   const bool at_max = !east;
   const T two = T(2.0);
   const T S0 = ((((((((f[0] + f[3]) + f[6]) + f[9]) + f[12]) + f[15]) + f[18]) + f[21]) + f[24]);
@@ -252,6 +254,7 @@ __device__ __forceinline__ void zouhe_p_d3q27(T* f, bool east, T rs) {
     uy = y0 + two * (((((-f[4]) + f[7]) - f[13]) + f[16]) - f[22] + f[25]);
     uz = z0 + two * (((((-f[10]) - f[13]) - f[16]) + f[19]) + f[22] + f[25]);
   }
+  if (normal_only) uy = uz = T(0);
   const T c49 = T(4.0 / 9.0), c19 = T(1.0 / 9.0), c136 = T(1.0 / 36.0);
   if (at_max) {
     f[1] = f[2] - c49 * ux;

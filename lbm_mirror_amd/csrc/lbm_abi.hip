@@ -177,6 +177,27 @@ int particle_storage(int storage, const lbm_particle_params& pp) {
   return (storage == LBM_F64 || pp.state_fp64) ? LBM_F64 : LBM_F32;
 }
 
+// normalise_spin_axis (integrate.hpp:115-131): the sign in 2D; in 3D the
+// axis over its length, in T (math::normalize, abstract/math.hpp:70-87: a
+// zero vector stays zero).  Results are exact in double.
+template <class T>
+void spin_axis_normalised(int D, const double* in, double* out) {
+  out[0] = out[1] = out[2] = 0.0;
+  if (D == 2) {
+    out[0] = T(in[0]) < T(0) ? -1.0 : 1.0;
+    return;
+  }
+  const T a[3] = {T(in[0]), T(in[1]), T(in[2])};
+  T dot = T(0);
+  for (int i = 0; i < 3; ++i) dot = T(dot + T(a[i] * a[i]));
+  if (dot <= T(0)) {
+    for (int i = 0; i < 3; ++i) out[i] = double(a[i]);
+    return;
+  }
+  const T len = std::sqrt(dot);
+  for (int i = 0; i < 3; ++i) out[i] = double(T(a[i] / len));
+}
+
 ParticleArgs particle_args(lbm_ctx* c) {
   ParticleArgs a{};
   const int pst = particle_storage(c->storage, c->pp);
@@ -222,6 +243,11 @@ ParticleArgs particle_args(lbm_ctx* c) {
   a.umac = c->u;
   // fixed ramp width of the HLBM gather and the FPLBM particle (hlbm.hpp:173, fplbm.hpp:193)
   if (a.hlbm || a.fplbm) a.eps = to_T(pst, 1.5);
+  a.fixed_axis = c->pp.fixed_axis ? 1 : 0;
+  if (pst == LBM_F64)
+    spin_axis_normalised<double>(c->D, c->pp.spin_axis, a.axis);
+  else
+    spin_axis_normalised<float>(c->D, c->pp.spin_axis, a.axis);
   return a;
 }
 
@@ -688,6 +714,8 @@ lbm_status lbm_particles_set(lbm_ctx* c, const lbm_particle* p, uint32_t n,
     return fail(LBM_ERR_INVALID_ARG, "coupling %d", prm->coupling);
   if (prm->coupling != LBM_COUPLING_PSM && prm->porosity_mode != LBM_POROSITY_SCATTER)
     return fail(LBM_ERR_NOT_SUPPORTED, "HLBM and FPLBM assign porosity per particle (scatter mode only)");
+  if (prm->fixed_axis && prm->coupling == LBM_COUPLING_FPLBM)
+    return fail(LBM_ERR_NOT_SUPPORTED, "FpLbmOneParticle integrates the particle itself; no fixed-axis variant");
   HIP_TRY(hipSetDevice(c->device));
   HIP_TRY(hipStreamSynchronize(c->stream));
   const int st = particle_storage(c->storage, *prm);
@@ -708,12 +736,14 @@ lbm_status lbm_particles_set(lbm_ctx* c, const lbm_particle* p, uint32_t n,
   }
   c->pp = *prm;
   if (c->pp.cell_capacity <= 0) c->pp.cell_capacity = 4;
-  // blocks per particle so that one block sweeps at most ~1024 AABB cells
+  // blocks per particle so that one block sweeps ~512 AABB cells (two per
+  // thread): a single large sphere (r 19, 69k cells) then spreads over 135
+  // blocks instead of waiting on a few
   double rmax = 0.0;
   for (uint32_t k = 0; k < n; ++k) rmax = std::max(rmax, p[k].radius);
   uint64_t box = 1;
   for (int d = 0; d < c->D; ++d) box *= uint64_t(2.0 * rmax + 3.0);
-  uint32_t bpp = uint32_t(std::min<uint64_t>(64, std::max<uint64_t>(1, (box + 1023) / 1024)));
+  uint32_t bpp = uint32_t(std::min<uint64_t>(256, std::max<uint64_t>(1, (box + 511) / 512)));
   if (bpp != c->bpp || !c->partial) {
     if (c->partial) HIP_TRY(hipFree(c->partial));
     c->partial = nullptr;

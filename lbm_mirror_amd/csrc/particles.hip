@@ -602,8 +602,22 @@ __global__ void k_integrate(ParticleArgs a) {
   if (a.fplbm && a.sub_steps == 0) return;
   const uint64_t steps = a.sub_steps == 0 ? 1 : a.sub_steps;
   const T ts = T(1) / T(steps);
-  for (int k = 0; k < D; ++k) P[PF_ACC + k] = P[PF_FORCE + k] / P[PF_MASS] + T(a.g[k]);
-  if (D == 2 && !a.fplbm) P[PF_ANG_ACC] = P[PF_TORQUE] / P[PF_INERTIA];
+  if (a.fixed_axis) {
+    // integrate_particle_fixed_axis (integrate.hpp:143-190): pinned (zero
+    // acceleration), angular acceleration along the axis from axis·torque
+    for (int k = 0; k < D; ++k) P[PF_ACC + k] = T(0);
+    if (D == 2) {
+      P[PF_ANG_ACC] = (T(a.axis[0]) * P[PF_TORQUE]) / P[PF_INERTIA];
+    } else {
+      T tau = T(0);
+      for (int k = 0; k < 3; ++k) tau = tau + T(a.axis[k]) * P[PF_TORQUE + k];
+      const T alpha = tau / P[PF_INERTIA];
+      for (int k = 0; k < 3; ++k) P[PF_ANG_ACC + k] = T(a.axis[k]) * alpha;
+    }
+  } else {
+    for (int k = 0; k < D; ++k) P[PF_ACC + k] = P[PF_FORCE + k] / P[PF_MASS] + T(a.g[k]);
+    if (D == 2 && !a.fplbm) P[PF_ANG_ACC] = P[PF_TORQUE] / P[PF_INERTIA];
+  }
   constexpr int R = (D == 2) ? 1 : 3;
   for (uint64_t st = 0; st < steps; ++st) {
     const T ts2 = ts * ts;

@@ -119,7 +119,8 @@ __device__ __forceinline__ void apply_op(T (&f)[L::Q], const OpDev* op, const St
                               ? op->rho_start + op->rho_slope * double(a.t)
                               : op->rho_final;
         if constexpr (Q == 9) zouhe_p_d2q9<T>(f, op->east != 0, T(rs));
-        if constexpr (Q == 27) zouhe_p_d3q27<T>(f, op->east != 0, T(rs));
+        if constexpr (Q == 27)
+          zouhe_p_d3q27<T>(f, op->east != 0, T(rs), (op->flags & LBM_OPF_ZOUHE_NORMAL_ONLY) != 0);
         break;
       }
       case LBM_BC_ZOUHE_VELOCITY_X: {

@@ -27,6 +27,8 @@
 //   particle AABB         modules/core/c++/particle/aabb.hpp:41-84
 //   DEM contact           modules/core/c++/particle/contact.hpp:76-567
 //   integrate / reset     modules/core/c++/particle/integrate.hpp:16-83,195-296
+//   ParticleAxisIntegrate modules/core/c++/particle/integrate.hpp:101-189,298-327
+//                         (normalise_spin_axis, math::normalize abstract/math.hpp:70-87)
 //   Verlet step           modules/core/c++/particle/particle.hpp:171-211
 //   spheroid mass         modules/core/c++/particle/particle.hpp:79-101
 //   cell list / couple    modules/core/c++/particle/cell_list.hpp:23-206,
@@ -373,15 +375,60 @@ void wall_contacts(const Part<T>& me, Part<T>& self, const uint64_t* extent,
   }
 }
 
+// verlet sub-steps shared by both integrators (particle.hpp:171-211)
+template <class T, int D>
+void verlet_steps(Part<T>& p, uint64_t sub_steps);
+
+// normalise_spin_axis (integrate.hpp:115-131): the sign in 2D; the axis over
+// its length in T in 3D (math::normalize: dot from zero, a zero vector kept)
+template <class T, int D>
+void spin_axis(const double* in, T* out) {
+  out[0] = out[1] = out[2] = T(0);
+  if (D == 2) {
+    out[0] = T(in[0]) < T(0) ? T(-1) : T(1);
+    return;
+  }
+  T a[3] = {T(in[0]), T(in[1]), T(in[2])};
+  T dot = T(0);
+  for (int i = 0; i < 3; ++i) dot = T(dot + T(a[i] * a[i]));
+  if (dot <= T(0)) {
+    for (int i = 0; i < 3; ++i) out[i] = a[i];
+    return;
+  }
+  T len = T(std::sqrt(dot));
+  for (int i = 0; i < 3; ++i) out[i] = T(a[i] / len);
+}
+
+// integrate_particle_fixed_axis (integrate.hpp:143-190): acceleration 0
+// (pinned), angular acceleration axis * (axis . torque) / I
+template <class T, int D>
+void integrate_fixed_axis(Part<T>& p, const T* axis, uint64_t sub_steps) {
+  for (int i = 0; i < D; ++i) p.acc[i] = T(0);
+  if (D == 2) {
+    p.ang_acc[0] = T(T(axis[0] * p.torque[0]) / p.inertia);
+  } else {
+    T tau = T(0);
+    for (int i = 0; i < 3; ++i) tau = T(tau + T(axis[i] * p.torque[i]));
+    T alpha = T(tau / p.inertia);
+    for (int i = 0; i < 3; ++i) p.ang_acc[i] = T(axis[i] * alpha);
+  }
+  verlet_steps<T, D>(p, sub_steps);
+}
+
 // integrate_particle + verlet_step_lambda (integrate.hpp:45-83,
 // particle.hpp:171-211)
 // keep_ang: FpLbmOneParticle sets only the linear acceleration (fplbm.hpp:233-236)
 template <class T, int D>
 void integrate(Part<T>& p, const T* body_acc, uint64_t sub_steps, bool keep_ang = false) {
-  uint64_t steps = sub_steps_of(sub_steps);
-  T ts = T(T(1) / static_cast<T>(steps));
   for (int i = 0; i < D; ++i) p.acc[i] = T(T(p.force[i] / p.mass) + body_acc[i]);
   if (D == 2 && !keep_ang) p.ang_acc[0] = T(p.torque[0] / p.inertia);
+  verlet_steps<T, D>(p, sub_steps);
+}
+
+template <class T, int D>
+void verlet_steps(Part<T>& p, uint64_t sub_steps) {
+  uint64_t steps = sub_steps_of(sub_steps);
+  T ts = T(T(1) / static_cast<T>(steps));
   const int R = (D == 2) ? 1 : 3;
   for (uint64_t s = 0; s < steps; ++s) {
     T ts2 = T(ts * ts);
@@ -686,8 +733,10 @@ struct Sim final : SimBase {
       f[7] = T(T(f[8] - T(c16 * rvx)) + T(h * T(f[3] - f[4])));
     }
   }
-  // ZouHePressureX<East> for D3Q27 (boundary.hpp:229-756); AtXMax = !East
-  static void zouhe_p_d3q27(T* f, bool east, T rs) {
+  // ZouHePressureX<East> for D3Q27 (boundary.hpp:229-756); AtXMax = !East.
+  // normal_only: LBM_OPF_ZOUHE_NORMAL_ONLY, the tangential terms dropped
+  // (no reference counterpart)
+  static void zouhe_p_d3q27(T* f, bool east, T rs, bool normal_only = false) {
     const bool at_max = !east;
     const T two = T(2.0);
     T S0 = T(T(T(T(T(T(T(T(f[0] + f[3]) + f[6]) + f[9]) + f[12]) + f[15]) + f[18]) + f[21]) + f[24]);
@@ -713,6 +762,7 @@ struct Sim final : SimBase {
       T z1 = T(T(T(T(T(T(-f[10]) - f[13]) - f[16]) + f[19]) + f[22]) + f[25]);
       uz = T(z0 + T(two * z1));
     }
+    if (normal_only) uy = uz = T(0);
     const T c49 = T(4.0 / 9.0), c19 = T(1.0 / 9.0), c136 = T(1.0 / 36.0);
     if (at_max) {
       f[1] = T(f[2] - T(c49 * ux));
@@ -782,7 +832,7 @@ struct Sim final : SimBase {
                          : op.rho_final;
         T rs = T(rsd);
         if (Q == 9) zouhe_p_d2q9(f, op.east != 0, rs);
-        if (Q == 27) zouhe_p_d3q27(f, op.east != 0, rs);
+        if (Q == 27) zouhe_p_d3q27(f, op.east != 0, rs, (op.flags & LBM_OPF_ZOUHE_NORMAL_ONLY) != 0);
         break;
       }
       case LBM_BC_ZOUHE_VELOCITY_X: {
@@ -1252,6 +1302,12 @@ struct Sim final : SimBase {
   void integrate_all() override {
     // FpLbmOneParticle integrates only when sub_steps > 0 (fplbm.hpp:238-244)
     if (pp.coupling == LBM_COUPLING_FPLBM && pp.sub_steps == 0) return;
+    if (pp.fixed_axis) {
+      T ax[3];
+      spin_axis<T, D>(pp.spin_axis, ax);
+      for (auto& p : parts) integrate_fixed_axis<T, D>(p, ax, pp.sub_steps);
+      return;
+    }
     T g[3] = {T(pp.body_acceleration[0]), T(pp.body_acceleration[1]), T(pp.body_acceleration[2])};
     const bool fp = pp.coupling == LBM_COUPLING_FPLBM;
     for (auto& p : parts) integrate<T, D>(p, g, pp.sub_steps, fp);

@@ -12,7 +12,8 @@
 //   create_spheroid_particle_group mass/inertia  particle/particle.hpp:19-116
 //   narrow_phase / particle_contact_response / accumulate_particle_contacts /
 //   accumulate_box_wall_contacts  particle/contact.hpp:124-512
-//   reset_particle_accumulators / integrate_particle  particle/integrate.hpp
+//   reset_particle_accumulators / integrate_particle / integrate_particle_fixed_axis
+//                                                      particle/integrate.hpp
 //   make_cell_list / build_cell_list / cell_porosity  particle/cell_list.hpp,
 //                                                       particle/couple.hpp
 //   write_vtk_file / write_csv_file   write_vtk.hpp, write_csv.hpp
@@ -166,7 +167,17 @@ void dem_t(lbm_particle* p, const lbm_particle_params* prm, const uint64_t* exte
       if (prm->walls) kl::accumulate_box_wall_contacts<T, D>(parts.at({k}), ext, dem, {prm->sub_steps});
     }
   }
-  if (integrate) {
+  if (integrate && prm->fixed_axis) {
+    // what component<..., cmpt::ParticleAxisIntegrate> does (integrate.hpp:298-327)
+    kl::data<kl::impl::spin_axis_schema<T, D>> axis;
+    if constexpr (D == 2) {
+      axis.at({}).set(prm->spin_axis[0]);
+    } else {
+      for (uint64_t d = 0; d < 3; ++d) axis.at({{d}}).set(prm->spin_axis[d]);
+    }
+    const auto unit = kl::impl::normalise_spin_axis<T, D>(axis);
+    for (uint64_t k = 0; k < PC; ++k) kl::integrate_particle_fixed_axis<T, D>(parts.at({k}), unit, {prm->sub_steps});
+  } else if (integrate) {
     kl::data<sch::LbmAcceleration<sch::Vector<T, D>>> acc;
     for (uint64_t d = 0; d < D; ++d) acc.at({{d}}).set(prm->body_acceleration[d]);
     for (uint64_t k = 0; k < PC; ++k) kl::integrate_particle<T, D>(parts.at({k}), acc, {prm->sub_steps});

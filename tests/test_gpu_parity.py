@@ -95,7 +95,8 @@ def test_c2_periodic_duct_guo(hip, orc, lat, ext, st):
 
 
 @pytest.mark.parametrize("st", [abi.F64, abi.F32])
-def test_d3q27_zouhe_pressure_channel(hip, orc, st):
+@pytest.mark.parametrize("zflags", [0, abi.OPF_ZOUHE_NORMAL_ONLY])
+def test_d3q27_zouhe_pressure_channel(hip, orc, st, zflags):
     nx, ny, nz = 20, 8, 8
     fl = np.ones((nz, ny, nx), np.uint8)
     fl[1:-1, 1:-1, 1:-1] = 2
@@ -104,9 +105,9 @@ def test_d3q27_zouhe_pressure_channel(hip, orc, st):
     ops = {1: abi.op_desc(boundary=abi.BC_BOUNCE_BACK),
            2: abi.op_desc(collision=abi.COLL_BGK, tau=0.7),
            3: abi.op_desc(boundary=abi.BC_ZOUHE_PRESSURE_X, collision=abi.COLL_BGK, east=1, tau=0.7,
-                          ramp=(1.0, 0.001 / 16, 1.001, 16)),
+                          ramp=(1.0, 0.001 / 16, 1.001, 16), flags=zflags),
            4: abi.op_desc(boundary=abi.BC_ZOUHE_PRESSURE_X, collision=abi.COLL_BGK, east=0, tau=0.7,
-                          rho=1.0)}
+                          rho=1.0, flags=zflags)}
     g, o = _pair(hip, orc, abi.D3Q27, st, (nx, ny, nz))
     _setup((g, o), fl.reshape(-1), ops)
     g.step(45)

@@ -268,6 +268,98 @@ def test_zouhe_pressure_d3q27_mass(orc):
             assert f3[k] == f[k]
 
 
+def test_zouhe_pressure_d3q27_tangential_gain(orc):
+    """The reference's D3Q27 reconstruction (boundary.hpp:539-754) pairs each
+    unknown with its mirror across the face (f5 <- f4, not f5 <- f8) but adds
+    the opposite-pair terms 6 w rho (c.u), tangential parts included.  At the
+    x=0 face the unknowns' y-moment is then the mirrors' plus
+    (2/9 + 4/36) rho u_y = rho u_y / 3, so after the boundary the cell holds
+    4/3 of the tangential momentum estimated from the known populations
+    (likewise for z).  A sign slip in any u_y or u_z term changes the factor.
+    This gain is why channels driven through these faces diverge once a
+    tangential flow develops (the 3D drivers at 32^3 after ~100 steps)."""
+    D, Q, c, w, _ = orc.lattice(abi.D3Q27)
+    c = np.asarray(c)
+    rng = np.random.default_rng(21)
+    for _ in range(10):
+        f = _rand_f(abi.D3Q27, rng)
+        rs = rng.uniform(0.98, 1.02)
+        f2, _, _, _ = orc.cell_op(abi.D3Q27, F64,
+                                  abi.op_desc(boundary=abi.BC_ZOUHE_PRESSURE_X, east=1, rho=rs), f)
+        for a in (1, 2):
+            zero = c[:, 0] == 0
+            minus = c[:, 0] == -1
+            est = np.sum(c[zero, a] * f[zero]) + 2 * np.sum(c[minus, a] * f[minus])
+            assert abs(np.sum(c[:, a] * f2) - 4.0 / 3.0 * est) < 1e-14
+
+
+def test_zouhe_pressure_d3q27_normal_only(orc):
+    """LBM_OPF_ZOUHE_NORMAL_ONLY (no reference counterpart): unknown = mirror
+    + 6 w_i rho u_x c_ix.  On both faces the density is rho_set exactly and
+    the tangential momentum is the known-population estimate (gain 1, where
+    the reference's terms give 4/3); the known populations are untouched."""
+    D, Q, c, w, _ = orc.lattice(abi.D3Q27)
+    c = np.asarray(c)
+    rng = np.random.default_rng(22)
+    for _ in range(10):
+        f = _rand_f(abi.D3Q27, rng)
+        rs = rng.uniform(0.98, 1.02)
+        for east, known in ((1, -1), (0, 1)):
+            f2, _, _, _ = orc.cell_op(abi.D3Q27, F64, abi.op_desc(
+                boundary=abi.BC_ZOUHE_PRESSURE_X, east=east, rho=rs, flags=abi.OPF_ZOUHE_NORMAL_ONLY), f)
+            assert abs(f2.sum() - rs) < 1e-14
+            zero, kn = c[:, 0] == 0, c[:, 0] == known
+            assert np.array_equal(f2[zero | kn], f[zero | kn])
+            for a in (1, 2):
+                est = np.sum(c[zero, a] * f[zero]) + 2 * np.sum(c[kn, a] * f[kn])
+                assert abs(np.sum(c[:, a] * f2) - est) < 1e-14
+
+
+def test_d3q27_pressure_channel_stable_with_normal_only(orc):
+    """The 32^3 channel of the 3D drivers (pressure inflow ramp 1 -> 1.001,
+    outflow 1, tau 0.8, interior started at u_x = 0.01): with the reference's
+    Zou/He it leaves the finite numbers within 200 steps; with the
+    normal-only variant it settles into a duct flow, finite, along +x, and
+    symmetric about the cross-section's diagonal y = z."""
+    DIM = 32
+    N = DIM ** 3
+    fl = np.ones((DIM, DIM, DIM), np.uint8)
+    fl[1:-1, 1:-1, 1:-1] = 2
+    fl[1:-1, 1:-1, 0] = 3
+    fl[1:-1, 1:-1, DIM - 1] = 4
+    interior = np.zeros((DIM, DIM, DIM), bool)
+    interior[1:-1, 1:-1, 1:-1] = True
+    interior = interior.reshape(-1)
+    eq0 = orc.equilibrium(abi.D3Q27, F64, 1.0, (0, 0, 0))
+    eq1 = orc.equilibrium(abi.D3Q27, F64, 1.0, (0.01, 0, 0))
+
+    def run(flags, steps):
+        o = orc.Sim(abi.D3Q27, F64, (DIM, DIM, DIM))
+        o.set_flags(fl.reshape(-1))
+        ops = {1: abi.op_desc(boundary=abi.BC_BOUNCE_BACK), 2: abi.op_desc(collision=abi.COLL_BGK, tau=0.8),
+               3: abi.op_desc(boundary=abi.BC_ZOUHE_PRESSURE_X, collision=abi.COLL_BGK, east=1, tau=0.8,
+                              ramp=(1.0, 0.001 / 256, 1.001, 256), flags=flags),
+               4: abi.op_desc(boundary=abi.BC_ZOUHE_PRESSURE_X, collision=abi.COLL_BGK, east=0, tau=0.8,
+                              rho=1.0, flags=flags)}
+        for k, op in ops.items():
+            o.set_operator(k, op)
+        o.set_populations(np.where(interior[None, :], eq1[:, None], eq0[:, None]))
+        u0 = np.zeros((3, N))
+        u0[0, interior] = 0.01
+        o.set_macros(np.ones(N), u0)
+        o.step(steps, threads=8)
+        return o.get_macros()[1]
+
+    assert not np.all(np.isfinite(run(0, 200)))
+    u = run(abi.OPF_ZOUHE_NORMAL_ONLY, 600)
+    assert np.all(np.isfinite(u))
+    ux = u[0].reshape(DIM, DIM, DIM)
+    core = ux[2:-2, 2:-2, 2:-2]
+    assert core.min() > 0 and np.abs(u).max() < 0.02
+    # mirror symmetry about y = z (the duct cross-section's diagonal)
+    assert np.allclose(ux, ux.transpose(1, 0, 2), rtol=1e-9, atol=1e-12)
+
+
 def test_equilibrium_boundary(orc):
     op = abi.op_desc(boundary=abi.BC_EQUILIBRIUM, rho_eq=1.2, vec=(0.015, 0.0, 0.0))
     f2, _, _, _ = orc.cell_op(abi.D2Q9, F64, op, np.zeros(9))

@@ -178,3 +178,88 @@ def test_cell_list_product_porosity_bitwise(ref, orc, st):
         s.psm_couple()
         assert np.array_equal(s.get_porosity(), out)
         assert s.overflow() == ovf
+
+
+@pytest.mark.parametrize("st", STS)
+@pytest.mark.parametrize("D", [2, 3])
+def test_fixed_axis_integrate_bitwise(ref, orc, st, D):
+    """ParticleAxisIntegrate (integrate.hpp:143-190, 298-327): pinned bodies
+    spinning about a fixed, unnormalised axis under a torque that changes
+    every step; the oracle equals the reference's integrate_particle_fixed_axis
+    (with its normalise_spin_axis) bit for bit, over 30 steps."""
+    rng = np.random.default_rng(7 + D)
+    ext = [64, 40, 40][:D]
+    axes = [(0.0, 1.0, 0.0), (0.3, -2.0, 0.7), (0.0, 0.0, 0.0)] if D == 3 else [(1.0,), (-0.2,), (3.0,)]
+    for axis in axes:
+        n = 2
+        parts = _rand_group(rng, n, D, st, orc)
+        for p in parts:  # at rest: x_old == x, which is what keeps a pinned body in place
+            for d in range(D):
+                p.position_old[d] = p.position[d]
+        prm = abi.particle_params(k_n=0.0, sub_steps=8, max_radius=2.0, spin_axis=axis)
+        s = orc.Sim(abi.D2Q9 if D == 2 else abi.D3Q19, st, ext)
+        s.particles_set(parts, prm)
+        arr = (abi.Particle * n)(*parts)
+        e3 = (C.c_uint64 * 3)(*(ext + [1] * (3 - D)))
+        start = [list(p.position) for p in parts]
+        for _ in range(30):
+            ps = s.particles_get()
+            for k, p in enumerate(ps):
+                for d in range(3 if D == 3 else 1):
+                    p.torque[d] = arr[k].torque[d] = _q(rng.uniform(-1, 1), st)
+                for d in range(D):
+                    p.force[d] = arr[k].force[d] = _q(rng.uniform(-1, 1), st)
+            s.particles_set(ps, prm)
+            s.integrate()
+            assert ref.ref_dem(D, st, n, arr, C.byref(prm), e3, 0, 0, 1) == 0
+            R = 1 if D == 2 else 3
+            for a, b in zip(s.particles_get(), arr):
+                for fld, m in (("position", D), ("position_old", D), ("acceleration", D),
+                               ("rotation", R), ("rotation_old", R), ("angular_acceleration", R)):
+                    assert list(getattr(a, fld))[:m] == list(getattr(b, fld))[:m], (axis, fld)
+        # pinned: the force is ignored and the body never moves
+        for a, p0 in zip(s.particles_get(), start):
+            assert list(a.position)[:D] == p0[:D]
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_fixed_axis_closed_form(orc, D):
+    """Constant torque tau about the axis: position Verlet from rest
+    (theta_old == theta) under constant angular acceleration alpha gives
+    theta_k = alpha dt^2 k (k + 1) / 2 after k sub-steps of dt, so after n
+    steps of s sub-steps theta = axis * alpha (n s)(n s + 1) / (2 s^2); an
+    off-axis torque component does not spin the body; a zero axis never
+    spins it."""
+    st = abi.F64
+    p = make_particle([20.0] * D, 2.0, 1.0, D, st, orc=orc)
+    axis = (0.0, 0.0, 2.0) if D == 3 else (-1.0,)
+    prm = abi.particle_params(k_n=0.0, sub_steps=4, max_radius=2.0, spin_axis=axis)
+    s = orc.Sim(abi.D2Q9 if D == 2 else abi.D3Q19, st, [40] * D)
+    s.particles_set([p], prm)
+    steps, tau = 25, 0.125
+    for _ in range(steps):
+        ps = s.particles_get()
+        if D == 3:
+            ps[0].torque[0], ps[0].torque[1], ps[0].torque[2] = 0.5, -0.25, tau
+        else:
+            ps[0].torque[0] = tau
+        s.particles_set(ps, prm)
+        s.integrate()
+    q = s.particles_get()[0]
+    alpha = tau / p.inertia
+    k = steps * 4
+    theta = alpha * k * (k + 1) / (2 * 4 ** 2)
+    if D == 3:
+        assert q.rotation[0] == 0.0 and q.rotation[1] == 0.0
+        assert np.isclose(q.rotation[2], theta, rtol=1e-12)
+    else:
+        assert np.isclose(q.rotation[0], -theta, rtol=1e-12)
+    zero = abi.particle_params(k_n=0.0, sub_steps=4, max_radius=2.0, spin_axis=(0.0, 0.0, 0.0))
+    if D == 3:
+        s2 = orc.Sim(abi.D3Q19, st, [40] * 3)
+        s2.particles_set([p], zero)
+        ps = s2.particles_get()
+        ps[0].torque[0] = ps[0].torque[1] = ps[0].torque[2] = 1.0
+        s2.particles_set(ps, zero)
+        s2.integrate()
+        assert list(s2.particles_get()[0].rotation) == [0.0, 0.0, 0.0]
