@@ -17,6 +17,9 @@
 //   make_cell_list / build_cell_list / cell_porosity  particle/cell_list.hpp,
 //                                                       particle/couple.hpp
 //   write_vtk_file / write_csv_file   write_vtk.hpp, write_csv.hpp
+//   converter, parse_args, load_lbm_config, init_lbm_env, output_directory
+//                                     converter.hpp, abstract/args.hpp, config.hpp,
+//                                     environment.hpp
 //   codec<Struct<Chunk...>, enc::Forst>::encode   abstract/forst.hpp, forst_chunk.hpp
 // The fluid components (BGK, BGKGuo, Stream, Zou/He, Psm, the PSM gather)
 // take Chunk field views that only the SYCL backend provides
@@ -34,6 +37,10 @@
 #include "modules/core/c++/particle/integrate.hpp"
 #include "modules/core/c++/particle/particle.hpp"
 #include "modules/core/c++/particle/porosity.hpp"
+#include "modules/core/c++/abstract/args.hpp"
+#include "modules/core/c++/config.hpp"
+#include "modules/core/c++/converter.hpp"
+#include "modules/core/c++/environment.hpp"
 #include "modules/core/c++/forst_chunk.hpp"
 #include "modules/core/c++/write_csv.hpp"
 #include "modules/core/c++/write_vtk.hpp"
@@ -300,6 +307,45 @@ int64_t write_dims(int shape, int full, const out_in& in, const char* dir, const
   return -1;
 }
 
+
+// ---- unit / config front end ---------------------------------------------------
+template <typename T>
+double convert_t(double dx, double dt, int what, double v) {
+  kl::converter<T> conv{{dx}, {dt}};
+  switch (what) {
+    case 0: return conv.meter_si_to_lbm(kl::data<sch::SiMeter<T>>{v}).handle().get();
+    case 1: return conv.meter_lbm_to_si(kl::data<sch::LbmMeter<T>>{v}).handle().get();
+    case 2: return conv.second_si_to_lbm(kl::data<sch::SiSecond<T>>{v}).handle().get();
+    case 3: return conv.velocity_si_to_lbm(kl::data<sch::SiVelocity<T>>{v}).handle().get();
+    case 4: return conv.acceleration_si_to_lbm(kl::data<sch::SiAcceleration<T>>{v}).handle().get();
+    case 5: return conv.kinematic_viscosity_si_to_lbm(kl::data<sch::SiKinematicViscosity<T>>{v}).handle().get();
+    case 6: return conv.template kinematic_viscosity_si_to_tau<sch::D2Q9>(kl::data<sch::SiKinematicViscosity<T>>{v}).handle().get();
+    case 7: return conv.template kinematic_viscosity_lbm_to_tau<sch::D2Q9>(kl::data<sch::LbmKinematicViscosity<T>>{v}).handle().get();
+    case 8: return conv.delta_x().handle().get();
+    case 9: return conv.delta_t().handle().get();
+    case 10: return conv.delta_v().handle().get();
+    case 11: return conv.delta_a().handle().get();
+    case 12: return conv.template kinematic_viscosity_si_to_tau<sch::D3Q27>(kl::data<sch::SiKinematicViscosity<T>>{v}).handle().get();
+  }
+  return 0.0;
+}
+
+// The drivers' argument struct (vawt_3d_hlbm_gpu/sim.cpp:44-52 and
+// dkt's coupling string) plus an integer positional pair, as the
+// reference's own args tests use (tests/args.cpp:33-58).
+using RefArgs = sch::Args<
+    sch::Struct<sch::Member<sch::String, "name">, sch::Member<sch::UInt64, "time_steps">,
+                sch::Member<sch::UInt64, "sub_steps">, sch::Member<sch::Float64, "axis_x">,
+                sch::Member<sch::Float32, "visc">, sch::Member<sch::Int64, "shift">,
+                sch::Member<sch::String, "coupling">, sch::Member<sch::Bool, "verbose">>,
+    sch::Tuple<sch::UInt64, sch::Float64>>;
+
+void put_text(char* out, uint64_t cap, std::string_view v) {
+  if (!out || cap == 0) return;
+  const uint64_t n = std::min<uint64_t>(cap - 1, v.size());
+  std::memcpy(out, v.data(), n);
+  out[n] = '\0';
+}
 }  // namespace
 
 extern "C" {
@@ -403,6 +449,83 @@ int64_t ref_write_fields(int shape, int st, int full, const double* vel, const d
                             : write_dims<sch::Float32>(shape, full, in, out_dir, name, t, what, buf, cap);
   if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return r;
+}
+
+double ref_convert(int st, double dx, double dt, int what, double v) {
+  return st == LBM_F64 ? convert_t<sch::Float64>(dx, dt, what, v) : convert_t<sch::Float32>(dx, dt, what, v);
+}
+
+// parse_args on RefArgs.  Returns 0 on success, else 1 + the error's
+// description and message in err.  nums: time_steps, sub_steps, axis_x,
+// visc, shift, verbose, pos0, pos1; text: "name\ncoupling\nprogram".
+int ref_parse_args(int argc, char** argv, double* nums, char* text, uint64_t text_cap, char* err,
+                   uint64_t err_cap) {
+  kl::data<RefArgs> a;
+  auto& n = a.template get<"args">();
+  n.template get<"name">().set(std::string{"default_name"});
+  n.template get<"time_steps">().set(256u);
+  n.template get<"sub_steps">().set(16u);
+  auto eov = kl::parse_args(a, argc, argv);
+  if (eov.is_error()) {
+    auto& e = eov.get_error();
+    std::string m = std::string{e.get_category()} + "|" + std::string{e.get_message()};
+    put_text(err, err_cap, m);
+    return 1;
+  }
+  nums[0] = double(n.template get<"time_steps">().get());
+  nums[1] = double(n.template get<"sub_steps">().get());
+  nums[2] = n.template get<"axis_x">().get();
+  nums[3] = double(n.template get<"visc">().get());
+  nums[4] = double(n.template get<"shift">().get());
+  nums[5] = n.template get<"verbose">().get() ? 1.0 : 0.0;
+  auto& pos = a.template get<"positionals">();
+  nums[6] = double(pos.template get<0u>().get());
+  nums[7] = pos.template get<1u>().get();
+  std::string t = std::string{n.template get<"name">().stl_view()} + "\n" +
+                  std::string{n.template get<"coupling">().stl_view()} + "\n" +
+                  std::string{a.template get<"program">().stl_view()};
+  put_text(text, text_cap, t);
+  return 0;
+}
+
+// load_lbm_config (config.hpp:22-50): 0 and the three values, else 1 and the
+// error's category|message.
+int ref_load_config(const char* path, int st, double* out, char* err, uint64_t err_cap) {
+  auto run = [&](auto zero) -> int {
+    using T = decltype(zero);
+    auto eo = kl::load_lbm_config<T, sch::D2Q9>(path);
+    if (eo.is_error()) {
+      auto& e = eo.get_error();
+      put_text(err, err_cap, std::string{e.get_category()} + "|" + std::string{e.get_message()});
+      return 1;
+    }
+    auto& c = eo.get_value();
+    out[0] = double(c.template get<"delta_x">().get());
+    out[1] = double(c.template get<"kinematic_viscosity">().get());
+    out[2] = double(c.template get<"delta_t">().get());
+    return 0;
+  };
+  return st == LBM_F64 ? run(sch::Float64{}) : run(sch::Float32{});
+}
+
+// init_lbm_env (environment.hpp:18-56) for an argument struct named `name`,
+// and output_directory (:63-89): lbm_dir, data_dir, name_dir, output_dir
+// separated by the unit separator 0x1f.
+int ref_lbm_env(const char* name, char* out, uint64_t cap, char* err, uint64_t err_cap) {
+  kl::data<RefArgs> a;
+  a.template get<"args">().template get<"name">().set(std::string{name});
+  auto eo = kl::init_lbm_env(a);
+  if (eo.is_error()) {
+    put_text(err, err_cap, std::string{eo.get_error().get_category()} + "|" +
+                               std::string{eo.get_error().get_message()});
+    return 1;
+  }
+  auto od = kl::output_directory();
+  if (od.is_error()) return 2;
+  auto& e = eo.get_value();
+  put_text(out, cap, e.lbm_dir.string() + "\x1f" + e.data_dir.string() + "\x1f" + e.name_dir.string() + "\x1f" +
+                         od.get_value().string());
+  return 0;
 }
 
 }  // extern "C"
