@@ -220,15 +220,21 @@ __device__ __forceinline__ void zouhe_p_d2q9(T* f, bool dir, T rs) {
   else
     rvx = (((f[0] + f[3]) + f[4]) + ((f[2] + f[6]) + f[8]) * T(2u)) - rs;
   const T c23 = T(2.0 / 3.0), c16 = T(1.0 / 6.0), h = T(0.5);
-  if (dir) {
-    f[2] = f[1] + c23 * rvx;
-    f[6] = (f[5] + c16 * rvx) + h * (f[3] - f[4]);
-    f[8] = (f[7] + c16 * rvx) + h * (f[4] - f[3]);
-  } else {
-    f[1] = f[2] - c23 * rvx;
-    f[5] = (f[6] - c16 * rvx) + h * (f[4] - f[3]);
-    f[7] = (f[8] - c16 * rvx) + h * (f[3] - f[4]);
-  }
+  // both faces' values, then a select per population: the compiler would
+  // otherwise merge the two branches' stores into one through a selected
+  // address and move the population array to scratch memory
+  const T e2 = f[1] + c23 * rvx;
+  const T e6 = (f[5] + c16 * rvx) + h * (f[3] - f[4]);
+  const T e8 = (f[7] + c16 * rvx) + h * (f[4] - f[3]);
+  const T w1 = f[2] - c23 * rvx;
+  const T w5 = (f[6] - c16 * rvx) + h * (f[4] - f[3]);
+  const T w7 = (f[8] - c16 * rvx) + h * (f[3] - f[4]);
+  f[1] = dir ? f[1] : w1;
+  f[2] = dir ? e2 : f[2];
+  f[5] = dir ? f[5] : w5;
+  f[6] = dir ? e6 : f[6];
+  f[7] = dir ? f[7] : w7;
+  f[8] = dir ? e8 : f[8];
 }
 
 // ZouHePressureX<East>, D3Q27 (boundary.hpp:256-754); the body is written for
@@ -285,15 +291,19 @@ __device__ __forceinline__ void zouhe_u_d2q9(T* f, bool east, T u0, T u1) {
   const T ds = east ? ((f[1] + f[5]) + f[7]) : ((f[2] + f[6]) + f[8]);
   const T r = (((f[0] + f[4]) + f[3]) + ds * T(2)) / (u0 + T(1.0));
   const T c23 = T(2.0 / 3.0), c16 = T(1.0 / 6.0), h = T(0.5f);
-  if (east) {
-    f[2] = f[1] + (c23 * r) * u0;
-    f[6] = (f[5] + (c16 * r) * u0) + (((f[3] - f[4]) + r * u1) * h);
-    f[8] = (f[7] + (c16 * r) * u0) + (((f[4] - f[3]) + r * u1) * h);
-  } else {
-    f[1] = f[2] + (c23 * r) * u0;
-    f[5] = (f[6] + (c16 * r) * u0) + (((f[4] - f[3]) + r * u1) * h);
-    f[7] = (f[8] + (c16 * r) * u0) + (((f[3] - f[4]) + r * u1) * h);
-  }
+  // both faces' values, then a select per population (see zouhe_p_d2q9)
+  const T e2 = f[1] + (c23 * r) * u0;
+  const T e6 = (f[5] + (c16 * r) * u0) + (((f[3] - f[4]) + r * u1) * h);
+  const T e8 = (f[7] + (c16 * r) * u0) + (((f[4] - f[3]) + r * u1) * h);
+  const T w1 = f[2] + (c23 * r) * u0;
+  const T w5 = (f[6] + (c16 * r) * u0) + (((f[4] - f[3]) + r * u1) * h);
+  const T w7 = (f[8] + (c16 * r) * u0) + (((f[3] - f[4]) + r * u1) * h);
+  f[1] = east ? f[1] : w1;
+  f[2] = east ? e2 : f[2];
+  f[5] = east ? f[5] : w5;
+  f[6] = east ? e6 : f[6];
+  f[7] = east ? f[7] : w7;
+  f[8] = east ? e8 : f[8];
 }
 
 // particle_porosity<..., ParticleSpheroid>::calculate (porosity.hpp:56-89).

@@ -51,8 +51,10 @@ template <class S, class T>
 struct Store;
 template <>
 struct Store<double, double> {
-  __device__ static double ld(const double* p, double) { return *p; }
-  __device__ static void st(double* p, double v, double) { *p = v; }
+  template <class P>
+  __device__ static double ld(P p, double) { return *p; }
+  template <class P>
+  __device__ static void st(P p, double v, double) { *p = v; }
   __device__ static double zero(double) { return 0.0; }
 };
 #ifndef LBMX_NT
@@ -60,11 +62,13 @@ struct Store<double, double> {
 #endif
 template <>
 struct Store<float, float> {
-  __device__ static float ld(const float* p, float) {
+  template <class P>
+  __device__ static float ld(P p, float) {
     if (LBMX_NT & 1) return __builtin_nontemporal_load(p);
     return *p;
   }
-  __device__ static void st(float* p, float v, float) {
+  template <class P>
+  __device__ static void st(P p, float v, float) {
     if (LBMX_NT & 2) __builtin_nontemporal_store(v, p);
     else *p = v;
   }
@@ -72,22 +76,42 @@ struct Store<float, float> {
 // Shifted 16-bit storage: the stored value is f_i − w_i.
 template <>
 struct Store<__half, float> {
-  __device__ static float ld(const __half* p, float w) { return __half2float(*p) + w; }
-  __device__ static void st(__half* p, float v, float w) { *p = __float2half(v - w); }
+  // as raw 16-bit words: the class type cannot be accessed through a
+  // global-typed pointer
+  template <class P>
+  __device__ static float ld(P p, float w) {
+    return __half2float(__ushort_as_half(*(const __attribute__((address_space(1))) unsigned short*)p)) + w;
+  }
+  template <class P>
+  __device__ static void st(P p, float v, float w) {
+    *(__attribute__((address_space(1))) unsigned short*)p = __half_as_ushort(__float2half(v - w));
+  }
 };
 template <>
 struct Store<__hip_bfloat16, float> {
-  __device__ static float ld(const __hip_bfloat16* p, float w) {
-    return __bfloat162float(*p) + w;
+  template <class P>
+  __device__ static float ld(P p, float w) {
+    return __bfloat162float(__ushort_as_bfloat16(*(const __attribute__((address_space(1))) unsigned short*)p)) + w;
   }
-  __device__ static void st(__hip_bfloat16* p, float v, float w) { *p = __float2bfloat16(v - w); }
+  template <class P>
+  __device__ static void st(P p, float v, float w) {
+    *(__attribute__((address_space(1))) unsigned short*)p = __bfloat16_as_ushort(__float2bfloat16(v - w));
+  }
 };
 
 // Boundary then collision for one cell (the drivers' switch(info)), writing
 // the macros when asked.  f holds the pre-collision populations on entry and
 // the post-collision ones on exit.
-template <class L, class T, bool WB>
-__device__ __forceinline__ void apply_op(T (&f)[L::Q], const OpDev* op, const StepArgs& a,
+// A field of the lattice, typed as global memory: accesses through it are
+// GLOBAL, not FLAT, also where the pointer was loaded from memory (batch
+// members).
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* gfield(const void* p) {
+  return (__attribute__((address_space(1))) T*)(p);
+}
+
+template <class L, class T, bool WB, class OpP>
+__device__ __forceinline__ void apply_op(T (&f)[L::Q], OpP op, const StepArgs& a,
                                          uint32_t cell, int x, int y, int z) {
   constexpr int Q = L::Q;
   {
@@ -125,7 +149,7 @@ __device__ __forceinline__ void apply_op(T (&f)[L::Q], const OpDev* op, const St
       }
       case LBM_BC_ZOUHE_VELOCITY_X: {
         if constexpr (Q == 9) {
-          const T* bv = static_cast<const T*>(a.bcvel);
+          const auto bv = gfield<const T>(a.bcvel);
           zouhe_u_d2q9<T>(f, op->east != 0, bv[cell], bv[a.N + cell]);
         }
         break;
@@ -141,7 +165,7 @@ __device__ __forceinline__ void apply_op(T (&f)[L::Q], const OpDev* op, const St
     if (coll != LBM_COLL_NONE) {
       T rho, u[3];
       T B = T(1);
-      T* por = static_cast<T*>(a.por);
+      auto por = gfield<T>(a.por);
       if (coll == LBM_COLL_PSM) {
         if (op->flags & LBM_OPF_PSM_STAMP_DISC) {
           const int xc[3] = {x, y, z};
@@ -156,8 +180,8 @@ __device__ __forceinline__ void apply_op(T (&f)[L::Q], const OpDev* op, const St
       }
       rho_u<L, T>(f, rho, u);
       if (a.write_macros) {
-        static_cast<T*>(a.rho)[cell] = rho;
-        T* um = static_cast<T*>(a.u);
+        gfield<T>(a.rho)[cell] = rho;
+        auto um = gfield<T>(a.u);
 #pragma unroll
         for (int d = 0; d < L::D; ++d) um[size_t(d) * a.N + cell] = u[d];
       }
@@ -166,7 +190,7 @@ __device__ __forceinline__ void apply_op(T (&f)[L::Q], const OpDev* op, const St
         bgk<L, T, WB>(f, rho, u, om);
       } else if (coll == LBM_COLL_BGK_GUO) {
         T F[3];
-        const T* cf = static_cast<const T*>(a.force);
+        const auto cf = gfield<const T>(a.force);
         const bool use_cf = op->flags & LBM_OPF_GUO_CELL_FORCE;
 #pragma unroll
         for (int d = 0; d < L::D; ++d) F[d] = (use_cf ? cf[size_t(d) * a.N + cell] : T(0)) + T(op->vec[d]);
@@ -175,7 +199,7 @@ __device__ __forceinline__ void apply_op(T (&f)[L::Q], const OpDev* op, const St
         // fplbm.hpp:60-133: BGKGuo's arithmetic with the cell force field and
         // u = m/rho; the macros hold rho and the momentum (FpLbmReset)
         T F[3] = {T(0), T(0), T(0)};
-        const T* cf = static_cast<const T*>(a.force);
+        const auto cf = gfield<const T>(a.force);
 #pragma unroll
         for (int d = 0; d < L::D; ++d) F[d] = cf[size_t(d) * a.N + cell];
         if (a.write_macros) {
@@ -185,7 +209,7 @@ __device__ __forceinline__ void apply_op(T (&f)[L::Q], const OpDev* op, const St
 #pragma unroll
             for (int d = 0; d < L::D; ++d)
               if (cdir<L>(j, d) != 0) m[d] = m[d] + T(cdir<L>(j, d)) * f[j];
-          T* um = static_cast<T*>(a.u);
+          auto um = gfield<T>(a.u);
 #pragma unroll
           for (int d = 0; d < L::D; ++d) um[size_t(d) * a.N + cell] = m[d];
         }
@@ -193,10 +217,10 @@ __device__ __forceinline__ void apply_op(T (&f)[L::Q], const OpDev* op, const St
       } else if (coll == LBM_COLL_HLBM) {
         // hlbm.hpp:62-105: velocity u*B + N(1-B)/D (N when D = 0); the macros
         // receive that velocity, written here after the plain moments
-        const T Bp = static_cast<const T*>(a.por)[cell];
+        const T Bp = gfield<const T>(a.por)[cell];
         const T flip = T(1.0) - Bp;
-        const T Dw = static_cast<const T*>(a.pd)[cell];
-        const T* pn = static_cast<const T*>(a.pn);
+        const T Dw = gfield<const T>(a.pd)[cell];
+        const auto pn = gfield<const T>(a.pn);
 #pragma unroll
         for (int d = 0; d < L::D; ++d) {
           const T Nd = pn[size_t(d) * a.N + cell];
@@ -204,7 +228,7 @@ __device__ __forceinline__ void apply_op(T (&f)[L::Q], const OpDev* op, const St
           u[d] = u[d] * Bp + term;
         }
         if (a.write_macros) {
-          T* um = static_cast<T*>(a.u);
+          auto um = gfield<T>(a.u);
 #pragma unroll
           for (int d = 0; d < L::D; ++d) um[size_t(d) * a.N + cell] = u[d];
         }
@@ -215,7 +239,7 @@ __device__ __forceinline__ void apply_op(T (&f)[L::Q], const OpDev* op, const St
           T m[3];
           opp_momentum<L, T>(f, m);
           const T sc = (op->flags & LBM_OPF_PSM_CELL_FORCE) ? B : T(1.0) - B;
-          T* fo = static_cast<T*>(a.force);
+          auto fo = gfield<T>(a.force);
 #pragma unroll
           for (int d = 0; d < L::D; ++d) fo[size_t(d) * a.N + cell] = m[d] * sc;
         }
@@ -230,11 +254,15 @@ __device__ __forceinline__ void collide_cell(T (&f)[L::Q], uint8_t fl, const Ste
   // When every active lane of the wavefront carries the same flag (the bulk
   // of any lattice), the operator is read with scalar loads and its kind is
   // a scalar branch; mixed wavefronts take the per-lane path.
+  // the operator table is global memory: typed so, its per-lane reads are
+  // GLOBAL loads also where the table pointer itself was loaded from memory
+  using GOp = const __attribute__((address_space(1))) OpDev;
+  GOp* ops = (GOp*)a.ops;
   const int fl0 = __builtin_amdgcn_readfirstlane(int(fl));
   if (__all(int(fl) == fl0)) {
-    if (fl0 < kMaxOps) apply_op<L, T, WB>(f, a.ops + fl0, a, cell, x, y, z);
+    if (fl0 < kMaxOps) apply_op<L, T, WB>(f, ops + fl0, a, cell, x, y, z);
   } else {
-    if (fl < kMaxOps) apply_op<L, T, WB>(f, a.ops + fl, a, cell, x, y, z);
+    if (fl < kMaxOps) apply_op<L, T, WB>(f, ops + fl, a, cell, x, y, z);
   }
 }
 
@@ -242,14 +270,14 @@ __device__ __forceinline__ void collide_cell(T (&f)[L::Q], uint8_t fl, const Ste
 #define LBMX_NT_EVEN 3
 #endif
 // Phase-0 accesses as nontemporal (bit 0: loads, bit 1: stores), FP32 only.
-template <class S, class T>
-__device__ __forceinline__ T ld_even(const S* p, T w) {
+template <class S, class T, class P>
+__device__ __forceinline__ T ld_even(P p, T w) {
   if constexpr (sizeof(S) == 4 && sizeof(T) == 4)
     if (LBMX_NT_EVEN & 1) return __builtin_nontemporal_load(p);
   return Store<S, T>::ld(p, w);
 }
-template <class S, class T>
-__device__ __forceinline__ void st_even(S* p, T v, T w) {
+template <class S, class T, class P>
+__device__ __forceinline__ void st_even(P p, T v, T w) {
   if constexpr (sizeof(S) == 4 && sizeof(T) == 4)
     if (LBMX_NT_EVEN & 2) {
       __builtin_nontemporal_store(v, p);
@@ -346,9 +374,14 @@ __device__ __forceinline__ void aa_row_cell(const StepArgs& a, int x, int y, int
                          uint32_t(x == nx - 1 ? 0 : x + 1)};
   const bool VX[3] = {x > 0 || px, true, x < nx - 1 || px};
   const uint32_t row = Y[1] + Z[1];
-  S* __restrict__ fb = static_cast<S*>(a.f);
+  // a batch member's pointers are loaded from memory, where the compiler
+  // cannot tell they address global memory: typing them global keeps the
+  // population accesses GLOBAL rather than FLAT (which wait on both the LDS
+  // and the vector-memory counters)
+  using GS = __attribute__((address_space(1))) S;
+  GS* __restrict__ fb = (GS*)static_cast<S*>(a.f);
   const size_t Np = a.Np;
-  auto rowp = [&](int plane, int iy, int iz) -> S* {
+  auto rowp = [&](int plane, int iy, int iz) -> GS* {
     return fb + (size_t(plane) * Np + (Y[iy] + Z[iz]));
   };
   // every neighbour of every lane in the lattice (or wrapped): no predication
@@ -372,7 +405,8 @@ __device__ __forceinline__ void aa_row_cell(const StepArgs& a, int x, int y, int
     }
   }
 
-  collide_cell<L, T, WB>(f, a.flags[row + X[1]], a, row + X[1], x, y, z);
+  collide_cell<L, T, WB>(f, ((const __attribute__((address_space(1))) uint8_t*)a.flags)[row + X[1]], a,
+                        row + X[1], x, y, z);
 
   if (!ODD) {
 #pragma unroll
@@ -646,6 +680,7 @@ __global__ __launch_bounds__(256) void k_aa_batch(BatchArgs b) {
     a.t = t;
     a.write_macros = b.write_macros != 0;
     if (b.write_macros == 2) a.u = b.u_prev[tl.member];
+
     const uint32_t nx = uint32_t(a.nx), sxy = nx * uint32_t(a.ny);
     for (uint32_t r = 0; r < b.rows; ++r) {
       const uint32_t c = tl.c0 + r * blockDim.x + threadIdx.x;
