@@ -286,8 +286,32 @@ __device__ __forceinline__ void st_even(P p, T v, T w) {
   Store<S, T>::st(p, v, w);
 }
 
+// Minimum waves per SIMD asked of the compiler for the fused step, per
+// lattice and compute type (1: no request).  D3Q27 FP32 phase 0 takes 182
+// VGPRs (2 waves per SIMD) unconstrained; held to 168 (3 waves) with a few
+// spills it runs 5% faster (23.2 -> 24.4 GLUPS at 256^3).  Asking D3Q19 for
+// 4 waves fits it in 128 VGPRs without spills but gains nothing measurable.
+#ifndef LBMX_WAVES_Q19
+#define LBMX_WAVES_Q19 1
+#endif
+#ifndef LBMX_WAVES_Q27
+#define LBMX_WAVES_Q27 3
+#endif
+template <class L, class T>
+struct StepWaves {
+  static constexpr int value = 1;
+};
+template <>
+struct StepWaves<D3Q19, float> {
+  static constexpr int value = LBMX_WAVES_Q19;
+};
+template <>
+struct StepWaves<D3Q27, float> {
+  static constexpr int value = LBMX_WAVES_Q27;
+};
 template <class L, class S, class T, bool WB, bool ODD>
-__global__ __launch_bounds__(256) void k_aa_step(StepArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(StepWaves<L, T>::value))) void k_aa_step(
+    StepArgs a) {
   constexpr int Q = L::Q;
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y * blockDim.y + threadIdx.y;
