@@ -59,11 +59,22 @@ def cpu_baseline(ext, steps=2, threads=None):
     t0 = time.perf_counter()
     s.step(steps, threads=threads)
     dt = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    s.step(1, threads=1)
+    dt1 = time.perf_counter() - t1
     cells = ext[0] * ext[1] * ext[2]
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            model = next((l.split(":", 1)[1].strip() for l in fh if l.startswith("model name")), "")
+    except OSError:
+        pass
     return {"value": round(cells * steps / dt / 1e6, 2), "unit": "MLUPS", "cores": threads,
             "kind": "port",
+            "one_thread": round(cells / dt1 / 1e6, 2), "nproc": os.cpu_count(), "cpu_model": model,
             "sample": f"D3Q19 FP32 {ext[0]}x{ext[1]}x{ext[2]}, {steps} steps after 1 warm-up, "
-                      f"oracle (reference arithmetic, OpenMP) on {threads} threads"}
+                      f"oracle (reference arithmetic, OpenMP) on {threads} threads; one_thread: 1 step "
+                      f"on 1 thread"}
 
 
 

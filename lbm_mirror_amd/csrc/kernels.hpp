@@ -112,6 +112,7 @@ struct ParticleArgs {
   const void* umac;       // D*N macro velocity field (FPLBM: m of the other cells)
   void* pn;         // D*N HLBM N sums (fluid field type)
   void* pd;         // N   HLBM D sums
+  int32_t fuse_finalize;  // the contact kernel finalizes the gather sums (full steps)
   int32_t fixed_axis;  // ParticleAxisIntegrate instead of ParticleIntegrate
   double axis[3];      // its normalised axis, T-rounded (2D: the sign in [0])
 };

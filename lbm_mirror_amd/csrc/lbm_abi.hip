@@ -266,6 +266,7 @@ lbm_status check_ready(lbm_ctx* c) {
 lbm_status particle_pass(lbm_ctx* c, uint64_t t) {
   (void)t;
   ParticleArgs a = particle_args(c);
+  a.fuse_finalize = 1;
   HIP_TRY(launch_psm_couple(c->lattice, c->storage, a, c->stream));
   HIP_TRY(launch_dem_contact(c->D, c->storage, a, c->stream));
   HIP_TRY(launch_particle_integrate(c->D, c->storage, a, c->stream));

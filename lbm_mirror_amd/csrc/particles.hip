@@ -1,7 +1,8 @@
 // Particle coupling kernels.
 //
 // One PSM coupling pass (lbm_psm_couple) is, in launch order:
-//   k_clear      porosity := 1 (PsmReset, psm.hpp:20-32), owners := 0
+//   k_reset_cells porosity := 1 (PsmReset, psm.hpp:20-32), owners := 0, HLBM N, D := 0,
+//                FPLBM force := 0
 //   k_owner      per particle and AABB cell: which particle a serial loop over
 //                particles would have written last (highest index), for the
 //                porosity and for the force field
@@ -10,7 +11,9 @@
 //                torque summed in FP64 (wavefront shuffles, then LDS, in a
 //                fixed order per block)
 //   k_finalize   per particle: fixed-order sum of the block partials; force
-//                reset + accumulate (integrate.hpp:195-212, psm.hpp:442-443)
+//                reset + accumulate (integrate.hpp:195-212, psm.hpp:442-443);
+//                inside lbm_step the contact kernel does this for its particle
+//                instead (fuse_finalize), one launch fewer per step
 //   [product mode] k_build_cells + k_product: cell list and the per-cell
 //                product porosity (cell_list.hpp:81-150, couple.hpp:26-60)
 // Then k_contact (contact.hpp:466-567) and k_integrate (integrate.hpp:45-83).
@@ -121,32 +124,27 @@ __device__ inline T porosity_at(const T* P, const int* x, T eps, T* rel) {
   return disc_porosity<T, D>(rel, P[PF_RADIUS], eps);
 }
 
-__global__ void k_clear(float* por_f, double* por_d, uint32_t* owner, uint32_t N, int reset_por) {
+
+
+// PsmReset / HlbmReset / FpLbmReset over every cell (psm.hpp:20-32,
+// hlbm.hpp:28-49, fplbm.hpp:21-58): porosity 1 (scatter mode), owners 0, and
+// where given N = 0, D = 0, force = 0
+template <class F>
+__global__ void k_reset_cells(F* por, uint32_t* owner, F* pn, F* pd, F* force, uint32_t N, int dims,
+                              int reset_por) {
   const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= N) return;
-  if (reset_por) {
-    if (por_d) por_d[c] = 1.0;
-    else por_f[c] = 1.0f;
-  }
+  if (reset_por) por[c] = F(1);
   owner[2 * c] = 0;
   owner[2 * c + 1] = 0;
+  if (pn) {
+    for (int d = 0; d < dims; ++d) pn[size_t(d) * N + c] = F(0);
+    pd[c] = F(0);
+  }
+  if (force)
+    for (int d = 0; d < dims; ++d) force[size_t(d) * N + c] = F(0);
 }
 
-// FpLbmReset's force = 0 over the lattice (fplbm.hpp:35-39)
-template <class F>
-__global__ void k_zero(F* v, size_t n) {
-  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < n) v[i] = F(0);
-}
-
-// HlbmReset (hlbm.hpp:28-49): N = 0, D = 0 (porosity in k_clear)
-template <class F>
-__global__ void k_hlbm_reset(F* pn, F* pd, uint32_t N, int dims) {
-  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= N) return;
-  pd[c] = F(0);
-  for (int d = 0; d < dims; ++d) pn[size_t(d) * N + c] = F(0);
-}
 
 // HlbmOneParticleStamp (hlbm.hpp:264-308): over the post-move AABB,
 // D += 1 - B and N += u_p (1 - B) with B the porosity of this step's gather
@@ -309,10 +307,10 @@ __global__ __launch_bounds__(256) void k_gather(ParticleArgs a) {
   }
 }
 
+// Particle p's gather partials summed in block order (FP64), then its force
+// (and torque where the coupling has one) set from them.
 template <class T, int D>
-__global__ void k_finalize(ParticleArgs a) {
-  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= a.n) return;
+__device__ __forceinline__ void finalize_one(const ParticleArgs& a, uint32_t p) {
   double s[6] = {0, 0, 0, 0, 0, 0};
   for (uint32_t k = 0; k < a.blocks_per_particle; ++k)
     for (int j = 0; j < 6; ++j) s[j] += a.partial[(size_t(p) * a.blocks_per_particle + k) * 6 + j];
@@ -327,6 +325,13 @@ __global__ void k_finalize(ParticleArgs a) {
     if (D == 2) P[PF_TORQUE] = T(s[3]);
     else for (int d = 0; d < 3; ++d) P[PF_TORQUE + d] = T(s[3 + d]);
   }
+}
+
+template <class T, int D>
+__global__ void k_finalize(ParticleArgs a) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= a.n) return;
+  finalize_one<T, D>(a, p);
 }
 
 // build_cell_list (cell_list.hpp:81-108): bins filled in particle-index order
@@ -509,6 +514,12 @@ __global__ __launch_bounds__(64) void k_contact(ParticleArgs a) {
   const uint32_t s = blockIdx.x;
   const int lane = int(threadIdx.x);
   if (s >= a.n) return;  // uniform per block
+  // inside a full step the gather's finalize of this particle runs here (the
+  // contact of s reads no other particle's force): one launch fewer per step
+  if (a.fuse_finalize) {
+    if (lane == 0) finalize_one<T, D>(a, s);
+    __syncthreads();
+  }
   T* parts = static_cast<T*>(a.parts);
   T* me = parts + size_t(s) * kPartFields;
   const T steps = T(a.sub_steps == 0 ? 1 : a.sub_steps);
@@ -637,19 +648,18 @@ __global__ void k_integrate(ParticleArgs a) {
 // ---------------------------------------------------------------- launchers
 template <class L, class S, class T, class F>
 static hipError_t couple_t(const ParticleArgs& a, hipStream_t s) {
-  const bool fdbl = sizeof(F) == 8;
   const uint32_t N = a.N;
   const bool scatter = a.porosity_mode == LBM_POROSITY_SCATTER;
-  hipLaunchKernelGGL(k_clear, dim3((N + 255) / 256), dim3(256), 0, s, fdbl ? nullptr : (float*)a.por,
-                     fdbl ? (double*)a.por : nullptr, a.owner, N, scatter ? 1 : 0);
-  if (a.hlbm) hipLaunchKernelGGL((k_hlbm_reset<F>), dim3((N + 255) / 256), dim3(256), 0, s, (F*)a.pn, (F*)a.pd, N, L::D);
-  if (a.fplbm)
-    hipLaunchKernelGGL((k_zero<F>), dim3((size_t(L::D) * N + 255) / 256), dim3(256), 0, s, (F*)a.force,
-                       size_t(L::D) * N);
+  // one pass over the cells: PsmReset's porosity, the owners, HlbmReset's N
+  // and D, FpLbmReset's force
+  hipLaunchKernelGGL((k_reset_cells<F>), dim3((N + 255) / 256), dim3(256), 0, s, (F*)a.por, a.owner,
+                     a.hlbm ? (F*)a.pn : nullptr, a.hlbm ? (F*)a.pd : nullptr, a.fplbm ? (F*)a.force : nullptr,
+                     N, L::D, scatter ? 1 : 0);
   const dim3 pg(a.n * a.blocks_per_particle);
   hipLaunchKernelGGL((k_owner<L, T>), pg, dim3(256), 0, s, a);
   hipLaunchKernelGGL((k_gather<L, S, T, F>), pg, dim3(256), 0, s, a);
-  hipLaunchKernelGGL((k_finalize<T, L::D>), dim3((a.n + 255) / 256), dim3(256), 0, s, a);
+  if (!a.fuse_finalize)
+    hipLaunchKernelGGL((k_finalize<T, L::D>), dim3((a.n + 255) / 256), dim3(256), 0, s, a);
   if (!scatter) {
     uint64_t bins = a.grid[0] * a.grid[1] * a.grid[2];
     hipLaunchKernelGGL((k_build_cells<T, L::D>), dim3(1), dim3(1024), 0, s, a, bins);
