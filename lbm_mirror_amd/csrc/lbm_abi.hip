@@ -84,6 +84,11 @@ struct lbm_ctx {
   int phase = 0;
   bool flags_set = false;
   LaunchShape shape;
+  // lbm_step replays: steps_per_graph steps captured once, keyed by the bytes
+  // of the launch arguments they captured (0 steps: every step launched)
+  uint32_t steps_per_graph = 32;
+  hipGraphExec_t step_graph = nullptr;
+  std::vector<uint8_t> graph_key;
   // particles
   uint32_t np = 0;
   bool pdbl = false;  // particle records are doubles
@@ -199,7 +204,8 @@ void spin_axis_normalised(int D, const double* in, double* out) {
 }
 
 ParticleArgs particle_args(lbm_ctx* c) {
-  ParticleArgs a{};
+  ParticleArgs a;
+  std::memset(&a, 0, sizeof a);  // padding too: the bytes key the step graphs
   const int pst = particle_storage(c->storage, c->pp);
   a.state64 = pst == LBM_F64 ? 1 : 0;
   a.f = c->f;
@@ -274,8 +280,9 @@ lbm_status particle_pass(lbm_ctx* c, uint64_t t) {
   return LBM_OK;
 }
 
-lbm_status fluid_pass(lbm_ctx* c, uint64_t t, bool write_macros) {
+StepArgs step_args(const lbm_ctx* c, uint64_t t, bool write_macros) {
   StepArgs a;
+  std::memset(&a, 0, sizeof a);  // padding too: the bytes key the step graphs
   a.f = c->f;
   a.flags = c->flags;
   a.ops = c->ops;
@@ -294,8 +301,66 @@ lbm_status fluid_pass(lbm_ctx* c, uint64_t t, bool write_macros) {
   a.periodic = c->periodic;
   a.t = t;
   a.write_macros = write_macros ? 1 : 0;
+  return a;
+}
+
+lbm_status fluid_pass(lbm_ctx* c, uint64_t t, bool write_macros) {
+  const StepArgs a = step_args(c, t, write_macros);
   HIP_TRY(launch_aa_step(c->lattice, c->storage, c->phase == 1, a, c->shape, c->stream));
   c->phase ^= 1;
+  return LBM_OK;
+}
+
+// The step index enters the kernels only through the Zou/He density ramps;
+// from the first step past every ramp on, all steps launch identical work.
+uint64_t ramp_end(const lbm_ctx* c) {
+  uint64_t e = 0;
+  for (int i = 0; i < kMaxOps; ++i)
+    if (c->has_op[i]) e = std::max<uint64_t>(e, c->ops_h[i].ramp_steps);
+  return e;
+}
+
+// Any step index past every ramp: what the captured kernels are given.
+constexpr uint64_t kGraphStep = uint64_t(1) << 62;
+
+template <class X>
+void append_bytes(std::vector<uint8_t>& k, const X& x) {
+  const uint8_t* p = reinterpret_cast<const uint8_t*>(&x);
+  k.insert(k.end(), p, p + sizeof(X));
+}
+
+// steps_per_graph full steps (particle pass and fluid pass, from phase 0)
+// launched as one graph, captured again whenever anything its launches
+// captured has changed
+lbm_status replay_steps(lbm_ctx* c, bool every) {
+  std::vector<uint8_t> key;
+  append_bytes(key, step_args(c, kGraphStep, every));
+  append_bytes(key, c->shape);
+  append_bytes(key, c->steps_per_graph);
+  append_bytes(key, c->np);
+  if (c->np > 0) {
+    ParticleArgs pa = particle_args(c);
+    pa.fuse_finalize = 1;
+    append_bytes(key, pa);
+  }
+  if (!c->step_graph || key != c->graph_key) {
+    if (c->step_graph) HIP_TRY(hipGraphExecDestroy(c->step_graph));
+    c->step_graph = nullptr;
+    hipGraph_t g = nullptr;
+    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    lbm_status s = LBM_OK;
+    for (uint32_t i = 0; i < c->steps_per_graph && !s; ++i) {
+      if (c->np > 0) s = particle_pass(c, kGraphStep);
+      if (!s) s = fluid_pass(c, kGraphStep, every);
+    }
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (s) return s;
+    HIP_TRY(e);
+    HIP_TRY(hipGraphInstantiate(&c->step_graph, g, nullptr, nullptr, 0));
+    HIP_TRY(hipGraphDestroy(g));
+    c->graph_key = key;  // capture flipped the phase an even number of times
+  }
+  HIP_TRY(hipGraphLaunch(c->step_graph, c->stream));
   return LBM_OK;
 }
 
@@ -378,6 +443,7 @@ lbm_status lbm_ctx_destroy(lbm_ctx* c) {
                   c->cl_overflow, c->owner};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  if (c->step_graph) (void)hipGraphExecDestroy(c->step_graph);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -414,6 +480,13 @@ lbm_status lbm_ctx_info(lbm_ctx* c, int32_t* lattice, int32_t* storage, uint64_t
   if (storage) *storage = c->storage;
   if (extent)
     for (int d = 0; d < 3; ++d) extent[d] = c->n[d];
+  return LBM_OK;
+}
+
+lbm_status lbm_set_step_graph(lbm_ctx* c, uint32_t steps) {
+  if (!c) return fail(LBM_ERR_INVALID_ARG, "null ctx");
+  if (steps % 2 || steps > 4096) return fail(LBM_ERR_INVALID_ARG, "steps per graph must be even and <= 4096");
+  c->steps_per_graph = steps;
   return LBM_OK;
 }
 
@@ -644,16 +717,28 @@ lbm_status lbm_step(lbm_ctx* c, uint64_t t0, uint64_t nsteps) {
   if (s) return s;
   HIP_TRY(hipSetDevice(c->device));
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
-  for (uint64_t k = 0; k < nsteps; ++k) {
+  // FPLBM reads the stored macro velocity of non-fluid cells every step
+  const bool every = c->np > 0 && c->pp.coupling == LBM_COUPLING_FPLBM;
+  const uint64_t ramps_done = ramp_end(c);
+  const uint64_t G = c->steps_per_graph;
+  uint64_t k = 0;
+  while (k < nsteps) {
     const uint64_t t = t0 + k;
+    // a whole graph of steps: past every ramp, from phase 0, and never the
+    // last step of the call (it writes the macros)
+    if (G > 0 && t >= ramps_done && c->phase == 0 && nsteps - k > G) {
+      s = replay_steps(c, every);
+      if (s) return s;
+      k += G;
+      continue;
+    }
     if (c->np > 0) {
       s = particle_pass(c, t);
       if (s) return s;
     }
-    // FPLBM reads the stored macro velocity of non-fluid cells every step
-    const bool every = c->np > 0 && c->pp.coupling == LBM_COUPLING_FPLBM;
     s = fluid_pass(c, t, k + 1 == nsteps || every);
     if (s) return s;
+    ++k;
   }
   HIP_TRY(hipEventRecord(c->ev1, c->stream));
   c->timed = true;

@@ -130,6 +130,10 @@ class Lattice:
     def sync(self):
         check(self.L.lbm_sync(self._h))
 
+    def set_step_graph(self, steps):
+        """Steps per hipGraph replay inside step() (0: launch every step)."""
+        check(self.L.lbm_set_step_graph(self._h, steps))
+
     def last_step_ms(self):
         m = C.c_double()
         check(self.L.lbm_last_step_ms(self._h, C.byref(m)))
