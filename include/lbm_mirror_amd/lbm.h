@@ -217,8 +217,9 @@ lbm_status lbm_set_macros(lbm_ctx* ctx, const double* rho, const double* u);
  * porosity reset, particle reset/gather/contact/integrate, collide, stream
  * (moving_poiseulle_particles_2d_psm_gpu/sim.cpp:176-280). */
 lbm_status lbm_step(lbm_ctx* ctx, uint64_t t0, uint64_t nsteps);
-/* Steps per hipGraph replay inside lbm_step (default 32; 0 launches every
- * step; even, at most 4096).  No reference counterpart.  Once every Zou/He
+/* Steps per hipGraph replay inside lbm_step (0 launches every step; even,
+ * at most 4096; default 32 up to 2^20 cells, 0 above, where a step's kernels
+ * are long enough that replays gain nothing).  No reference counterpart.  Once every Zou/He
  * density ramp has ended the step index no longer enters any kernel, so whole
  * runs of steps are replayed from one captured graph (recaptured whenever
  * an argument it captured changes); results are bit-identical either way. */

@@ -86,7 +86,7 @@ struct lbm_ctx {
   LaunchShape shape;
   // lbm_step replays: steps_per_graph steps captured once, keyed by the bytes
   // of the launch arguments they captured (0 steps: every step launched)
-  uint32_t steps_per_graph = 32;
+  uint32_t steps_per_graph = 0;
   hipGraphExec_t step_graph = nullptr;
   std::vector<uint8_t> graph_key;
   // particles
@@ -430,6 +430,10 @@ lbm_status lbm_ctx_create(lbm_ctx** out, int32_t lattice, int32_t storage,
   while (bx < 256 && uint64_t(bx) < n[0]) bx *= 2;
   c->shape.bx = bx;
   c->shape.by = 256 / bx;
+  // step graphs pay off where launches are a visible share of a step; past
+  // 2^20 cells (a 200 µs step at 8 M cells) they change nothing but cost a
+  // capture
+  c->steps_per_graph = N <= (uint64_t(1) << 20) ? 32u : 0u;
   *out = c;
   return LBM_OK;
 }
