@@ -159,14 +159,22 @@ class component<T, Desc, cmpt::ZouHePressureX<East>> {
   explicit component(double rho) : r0_(rho), s_(0), r1_(rho), n_(0) {}
   component(double rho_start, double slope, double rho_final, uint64_t ramp_steps)
       : r0_(rho_start), s_(slope), r1_(rho_final), n_(ramp_steps) {}
+  // D3Q27 only: LBM_OPF_ZOUHE_NORMAL_ONLY, the reconstruction without its
+  // tangential terms (no reference counterpart; keeps 3D channels stable)
+  component& normal_only(bool on = true) {
+    flags_ = on ? LBM_OPF_ZOUHE_NORMAL_ONLY : 0u;
+    return *this;
+  }
   void fill(lbm_op_desc& d) const {
     d.boundary = LBM_BC_ZOUHE_PRESSURE_X;
     d.east = East ? 1 : 0;
     d.rho_start = r0_; d.rho_slope = s_; d.rho_final = r1_; d.ramp_steps = n_;
+    d.flags |= flags_;
   }
  private:
   double r0_, s_, r1_;
   uint64_t n_;
+  uint32_t flags_ = 0;
 };
 
 template <typename T, typename Desc, bool East>

@@ -209,3 +209,18 @@ def test_against_reference_build_fresh_cases(tmp_path):
         want = _args_expect({"ok": r == 0, "nums": [float(x).hex() for x in nums], "text": txt.value.decode(),
                              "error": err.value.decode()})
         assert _match(_py_args(argv), want), (argv, _py_args(argv), want)
+
+
+def test_vawt_example_rejects_bad_arguments(tmp_path):
+    """The rotor example's command line goes through the argument parser
+    before any device call: a negative step count or an unknown name ends the
+    program with status 1 and a message, without touching a GPU."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "lbm_mirror_amd", "_build", "vawt_3d")
+    if not os.path.exists(exe):
+        pytest.skip("vawt_3d example not built")
+    env = dict(os.environ, HOME=str(tmp_path))
+    for argv in (["--time_steps", "-3"], ["--axis_q", "1"], ["--stable_faces", "maybe"]):
+        r = subprocess.run([exe, *argv], capture_output=True, text=True, timeout=60, env=env)
+        assert r.returncode == 1 and r.stderr, argv
+    assert not os.path.exists(tmp_path / ".lbm")
