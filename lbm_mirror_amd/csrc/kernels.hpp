@@ -101,7 +101,7 @@ struct ParticleArgs {
   uint64_t* cl_count;
   uint64_t* cl_slot;
   uint64_t* cl_overflow;
-  uint32_t* owner;  // per-cell owner (scatter mode)
+  uint32_t* owner;  // per-cell owners (more than kLocalOwnerMax particles), else null
   double bin;       // bin size (T-rounded)
   uint64_t grid[3];
   uint32_t cap;
@@ -116,6 +116,10 @@ struct ParticleArgs {
   int32_t fixed_axis;  // ParticleAxisIntegrate instead of ParticleIntegrate
   double axis[3];      // its normalised axis, T-rounded (2D: the sign in [0])
 };
+
+// Up to this many particles the gather decides cell ownership per block (no
+// per-cell owner array, no owner pass); above it the owner pass runs.
+constexpr uint32_t kLocalOwnerMax = 1024;
 
 // Per-particle record layout (T values)
 enum PartField {

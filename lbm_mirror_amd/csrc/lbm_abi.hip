@@ -237,7 +237,7 @@ ParticleArgs particle_args(lbm_ctx* c) {
   a.cl_count = c->cl_count;
   a.cl_slot = c->cl_slot;
   a.cl_overflow = c->cl_overflow;
-  a.owner = c->owner;
+  a.owner = c->np > kLocalOwnerMax ? c->owner : nullptr;
   a.bin = c->bin;
   for (int d = 0; d < 3; ++d) a.grid[d] = c->grid[d];
   a.cap = uint32_t(c->pp.cell_capacity);
@@ -849,7 +849,8 @@ lbm_status lbm_particles_set(lbm_ctx* c, const lbm_particle* p, uint32_t n,
       s = ensure_hlbm_fields(c);
       if (s) return s;
     }
-    if (!c->owner) HIP_TRY(alloc_zero((void**)&c->owner, size_t(c->N) * 2 * sizeof(uint32_t), c->stream));
+    if (!c->owner && n > kLocalOwnerMax)
+      HIP_TRY(alloc_zero((void**)&c->owner, size_t(c->N) * 2 * sizeof(uint32_t), c->stream));
   }
   // cell list (make_cell_list, particle/cell_list.hpp:185-206) in T
   {

@@ -3,9 +3,11 @@
 // One PSM coupling pass (lbm_psm_couple) is, in launch order:
 //   k_reset_cells porosity := 1 (PsmReset, psm.hpp:20-32), owners := 0, HLBM N, D := 0,
 //                FPLBM force := 0
-//   k_owner      per particle and AABB cell: which particle a serial loop over
-//                particles would have written last (highest index), for the
-//                porosity and for the force field
+//   k_owner      [more than kLocalOwnerMax particles] per particle and AABB
+//                cell: which particle a serial loop over particles would have
+//                written last (highest index), for the porosity and for the
+//                force field; with fewer particles each gather block decides
+//                this itself from the boxes that overlap its particle's
 //   k_gather     per particle and AABB cell: porosity, momentum exchange
 //                (psm.hpp:304-424), owner writes, and the particle's force and
 //                torque summed in FP64 (wavefront shuffles, then LDS, in a
@@ -108,14 +110,18 @@ __device__ inline Box<T, D> aabb_of(const T* P, const ParticleArgs& a) {
   return b;
 }
 
+// The k-th cell of the box, x fastest: neighbouring lanes take neighbouring
+// cells of one lattice row, so the field and population accesses of a
+// wavefront fall in runs of contiguous addresses (the box has no order of its
+// own: the force sums are FP64 and the owners are maxima).
 template <class T, int D>
 __device__ inline void box_cell(const Box<T, D>& b, uint32_t k, int* x) {
-  const uint32_t sz = uint32_t(b.hi[2] - b.lo[2]);
+  const uint32_t sx = uint32_t(b.hi[0] - b.lo[0]);
   const uint32_t sy = uint32_t(b.hi[1] - b.lo[1]);
-  x[2] = b.lo[2] + int(k % sz);
-  k /= sz;
+  x[0] = b.lo[0] + int(k % sx);
+  k /= sx;
   x[1] = b.lo[1] + int(k % sy);
-  x[0] = b.lo[0] + int(k / sy);
+  x[2] = b.lo[2] + int(k / sy);
 }
 
 template <class T, int D>
@@ -135,8 +141,10 @@ __global__ void k_reset_cells(F* por, uint32_t* owner, F* pn, F* pd, F* force, u
   const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= N) return;
   if (reset_por) por[c] = F(1);
-  owner[2 * c] = 0;
-  owner[2 * c + 1] = 0;
+  if (owner) {
+    owner[2 * c] = 0;
+    owner[2 * c + 1] = 0;
+  }
   if (pn) {
     for (int d = 0; d < dims; ++d) pn[size_t(d) * N + c] = F(0);
     pd[c] = F(0);
@@ -201,6 +209,83 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
+// Ownership without the per-cell owner array (a.owner == nullptr, at most
+// kLocalOwnerMax particles): a cell of particle p's box takes p's porosity if
+// no particle q > p has the cell in its box, and p's force if p is solid there
+// (B < 1) and no q > p is solid there -- what the atomic maxima of k_owner
+// decide.  Each gather block first lists the q > p whose boxes intersect p's
+// (LDS, kOverlapCap entries); a block whose list overflows tests every q > p.
+constexpr int kOverlapCap = 256;
+
+template <class T, int D>
+__device__ __forceinline__ bool box_has(const Box<T, D>& q, const int* x) {
+  for (int d = 0; d < D; ++d)
+    if (x[d] < q.lo[d] || x[d] >= q.hi[d]) return false;
+  return true;
+}
+
+template <class T, int D>
+struct LocalOwners {
+  int* cnt;        // LDS: entries found (may exceed kOverlapCap)
+  uint32_t* q;     // LDS: particle indices
+  int (*box)[6];   // LDS: their boxes (lo, hi)
+
+  // every thread of the block calls this once
+  __device__ void build(const ParticleArgs& a, uint32_t p, const Box<T, D>& b) const {
+    if (threadIdx.x == 0) *cnt = 0;
+    __syncthreads();
+    for (uint32_t j = p + 1 + threadIdx.x; j < a.n; j += blockDim.x) {
+      const Box<T, D> o = aabb_of<T, D>(static_cast<const T*>(a.parts) + size_t(j) * kPartFields, a);
+      bool hit = o.count > 0;
+      for (int d = 0; d < D; ++d) hit = hit && o.lo[d] < b.hi[d] && b.lo[d] < o.hi[d];
+      if (hit) {
+        const int e = atomicAdd(cnt, 1);
+        if (e < kOverlapCap) {
+          q[e] = j;
+          for (int d = 0; d < 3; ++d) {
+            box[e][d] = o.lo[d];
+            box[e][3 + d] = o.hi[d];
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  // (p owns the cell's porosity, p owns the cell's force); solid: p's B < 1
+  __device__ void test(const ParticleArgs& a, uint32_t p, const int* x, bool solid, T eps, bool& own_box,
+                       bool& own_solid) const {
+    own_box = true;
+    own_solid = solid;
+    const int n = *cnt;
+    if (n <= kOverlapCap) {
+      for (int e = 0; e < n; ++e) {
+        bool in = true;
+        for (int d = 0; d < D; ++d) in = in && x[d] >= box[e][d] && x[d] < box[e][3 + d];
+        if (!in) continue;
+        own_box = false;
+        if (own_solid) {
+          T rel[3];
+          const T* Pq = static_cast<const T*>(a.parts) + size_t(q[e]) * kPartFields;
+          if (porosity_at<T, D>(Pq, x, eps, rel) < T(1.0)) own_solid = false;
+        }
+        if (!own_solid) return;
+      }
+      return;
+    }
+    for (uint32_t j = p + 1; j < a.n; ++j) {
+      const T* Pq = static_cast<const T*>(a.parts) + size_t(j) * kPartFields;
+      if (!box_has<T, D>(aabb_of<T, D>(Pq, a), x)) continue;
+      own_box = false;
+      if (own_solid) {
+        T rel[3];
+        if (porosity_at<T, D>(Pq, x, eps, rel) < T(1.0)) own_solid = false;
+      }
+      if (!own_solid) return;
+    }
+  }
+};
+
 // S: population storage, T: particle arithmetic, F: fluid field type
 template <class L, class S, class T, class F>
 __global__ __launch_bounds__(256) void k_gather(ParticleArgs a) {
@@ -222,6 +307,12 @@ __global__ __launch_bounds__(256) void k_gather(ParticleArgs a) {
   F* por = static_cast<F*>(a.por);
   F* fout = static_cast<F*>(a.force);
   const bool scatter = a.porosity_mode == LBM_POROSITY_SCATTER;
+  const bool local = a.owner == nullptr;
+  __shared__ int lo_cnt;
+  __shared__ uint32_t lo_q[kOverlapCap];
+  __shared__ int lo_box[kOverlapCap][6];
+  const LocalOwners<T, D> lo{&lo_cnt, lo_q, lo_box};
+  if (local) lo.build(a, p, b);  // uniform per block
   double acc[6] = {0, 0, 0, 0, 0, 0};
   for (uint32_t k = chunk * blockDim.x + threadIdx.x; k < b.count;
        k += a.blocks_per_particle * blockDim.x) {
@@ -230,7 +321,14 @@ __global__ __launch_bounds__(256) void k_gather(ParticleArgs a) {
     T rel[3] = {T(0), T(0), T(0)};
     const T B = porosity_at<T, D>(P, x, eps, rel);
     const uint32_t c = uint32_t(x[0]) + uint32_t(a.nx) * (uint32_t(x[1]) + uint32_t(a.ny) * uint32_t(x[2]));
-    if (scatter && a.owner[2 * c] == p + 1) por[c] = F(B);
+    bool own_box, own_solid;
+    if (local) {
+      lo.test(a, p, x, !a.fplbm && B < T(1.0), eps, own_box, own_solid);
+    } else {
+      own_box = a.owner[2 * c] == p + 1;
+      own_solid = !a.fplbm && B < T(1.0) && a.owner[2 * c + 1] == p + 1;
+    }
+    if (scatter && own_box) por[c] = F(B);
     if (a.fplbm) {
       // FpLbmOneParticle (fplbm.hpp:204-224): m of a flag-2 cell from its
       // populations (FpLbmReset's compute_rho_momentum), else the stored macro
@@ -252,7 +350,7 @@ __global__ __launch_bounds__(256) void k_gather(ParticleArgs a) {
       const T flip = one - B;
       T fc[3] = {T(0), T(0), T(0)};
       for (int d = 0; d < D; ++d) fc[d] = (vp[d] - mom[d]) * flip;
-      if (a.owner[2 * c] == p + 1)
+      if (own_box)
         for (int d = 0; d < D; ++d) fout[size_t(d) * a.N + c] = F(fc[d]);
       for (int d = 0; d < D; ++d) acc[d] -= double(fc[d]);
       continue;
@@ -278,7 +376,7 @@ __global__ __launch_bounds__(256) void k_gather(ParticleArgs a) {
     const T flip = a.hlbm ? one : one - B;
     T fc[3] = {T(0), T(0), T(0)};
     for (int d = 0; d < D; ++d) fc[d] = mom[d] * flip;
-    if (a.owner[2 * c + 1] == p + 1)
+    if (own_solid)
       for (int d = 0; d < D; ++d) fout[size_t(d) * a.N + c] = F(fc[d]);
     for (int d = 0; d < D; ++d) acc[d] -= double(fc[d]);
     if (D == 2) {
@@ -656,7 +754,7 @@ static hipError_t couple_t(const ParticleArgs& a, hipStream_t s) {
                      a.hlbm ? (F*)a.pn : nullptr, a.hlbm ? (F*)a.pd : nullptr, a.fplbm ? (F*)a.force : nullptr,
                      N, L::D, scatter ? 1 : 0);
   const dim3 pg(a.n * a.blocks_per_particle);
-  hipLaunchKernelGGL((k_owner<L, T>), pg, dim3(256), 0, s, a);
+  if (a.owner) hipLaunchKernelGGL((k_owner<L, T>), pg, dim3(256), 0, s, a);
   hipLaunchKernelGGL((k_gather<L, S, T, F>), pg, dim3(256), 0, s, a);
   if (!a.fuse_finalize)
     hipLaunchKernelGGL((k_finalize<T, L::D>), dim3((a.n + 255) / 256), dim3(256), 0, s, a);
