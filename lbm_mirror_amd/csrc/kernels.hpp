@@ -113,6 +113,9 @@ struct ParticleArgs {
   void* pn;         // D*N HLBM N sums (fluid field type)
   void* pd;         // N   HLBM D sums
   int32_t fuse_finalize;  // the contact kernel finalizes the gather sums (full steps)
+  int32_t fuse_integrate; // the contact kernel integrates its particle (full steps):
+                          // partners are read from snap, written by the gather
+  void* snap;             // n records: the particles as the gather saw them
   int32_t fixed_axis;  // ParticleAxisIntegrate instead of ParticleIntegrate
   double axis[3];      // its normalised axis, T-rounded (2D: the sign in [0])
 };

@@ -93,6 +93,7 @@ struct lbm_ctx {
   uint32_t np = 0;
   bool pdbl = false;  // particle records are doubles
   void* parts = nullptr;
+  void* snap = nullptr;  // the records as the gather saw them (partners of the fused contact)
   double* f64 = nullptr;
   double* t64 = nullptr;
   double* partial = nullptr;
@@ -219,6 +220,7 @@ ParticleArgs particle_args(lbm_ctx* c) {
   a.periodic = c->periodic;
   a.phase = c->phase;
   a.parts = c->parts;
+  a.snap = c->snap;
   a.n = c->np;
   a.f64 = c->f64;
   a.t64 = c->t64;
@@ -273,9 +275,9 @@ lbm_status particle_pass(lbm_ctx* c, uint64_t t) {
   (void)t;
   ParticleArgs a = particle_args(c);
   a.fuse_finalize = 1;
+  a.fuse_integrate = 1;
   HIP_TRY(launch_psm_couple(c->lattice, c->storage, a, c->stream));
   HIP_TRY(launch_dem_contact(c->D, c->storage, a, c->stream));
-  HIP_TRY(launch_particle_integrate(c->D, c->storage, a, c->stream));
   if (a.hlbm) HIP_TRY(launch_hlbm_stamp(c->D, c->storage, a, c->stream));
   return LBM_OK;
 }
@@ -443,7 +445,7 @@ lbm_status lbm_ctx_destroy(lbm_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   void* ptrs[] = {c->f, c->flags, c->ops, c->rho, c->u, c->force, c->por, c->bcvel, c->pn, c->pd,
-                  c->parts, c->f64, c->t64, c->partial, c->cl_count, c->cl_slot,
+                  c->parts, c->snap, c->f64, c->t64, c->partial, c->cl_count, c->cl_slot,
                   c->cl_overflow, c->owner};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -812,14 +814,15 @@ lbm_status lbm_particles_set(lbm_ctx* c, const lbm_particle* p, uint32_t n,
   const bool dbl = compute_double(st);
   // (re)allocate
   if (n != c->np || dbl != c->pdbl) {
-    void* ptrs[] = {c->parts, c->f64, c->t64, c->partial};
+    void* ptrs[] = {c->parts, c->snap, c->f64, c->t64, c->partial};
     for (void* q : ptrs)
       if (q) HIP_TRY(hipFree(q));
-    c->parts = nullptr;
+    c->parts = c->snap = nullptr;
     c->f64 = c->t64 = c->partial = nullptr;
     c->np = 0;
     if (n > 0) {
       HIP_TRY(alloc_zero(&c->parts, size_t(n) * kPartFields * (dbl ? 8 : 4), c->stream));
+      HIP_TRY(alloc_zero(&c->snap, size_t(n) * kPartFields * (dbl ? 8 : 4), c->stream));
       HIP_TRY(alloc_zero((void**)&c->f64, size_t(n) * 3 * sizeof(double), c->stream));
       HIP_TRY(alloc_zero((void**)&c->t64, size_t(n) * 3 * sizeof(double), c->stream));
     }

@@ -18,7 +18,9 @@
 //                instead (fuse_finalize), one launch fewer per step
 //   [product mode] k_build_cells + k_product: cell list and the per-cell
 //                product porosity (cell_list.hpp:81-150, couple.hpp:26-60)
-// Then k_contact (contact.hpp:466-567) and k_integrate (integrate.hpp:45-83).
+// Then k_contact (contact.hpp:466-567) and k_integrate (integrate.hpp:45-83);
+// inside lbm_step the contact kernel integrates its particle itself
+// (fuse_integrate), reading its partners from the records the gather saved.
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
@@ -294,6 +296,8 @@ __global__ __launch_bounds__(256) void k_gather(ParticleArgs a) {
   const uint32_t chunk = blockIdx.x % a.blocks_per_particle;
   if (p >= a.n) return;  // uniform per block
   const T* P = static_cast<const T*>(a.parts) + size_t(p) * kPartFields;
+  if (a.fuse_integrate && chunk == 0 && threadIdx.x < kPartFields)
+    static_cast<T*>(a.snap)[size_t(p) * kPartFields + threadIdx.x] = P[threadIdx.x];
   const Box<T, D> b = aabb_of<T, D>(P, a);
   const T eps = T(a.eps);
   const T one = T(1.0);
@@ -618,8 +622,10 @@ __global__ __launch_bounds__(64) void k_contact(ParticleArgs a) {
     if (lane == 0) finalize_one<T, D>(a, s);
     __syncthreads();
   }
-  T* parts = static_cast<T*>(a.parts);
-  T* me = parts + size_t(s) * kPartFields;
+  // fused with the integrate, this block moves its particle while others
+  // still read it: partners then come from the gather's snapshot
+  const T* parts = static_cast<const T*>(a.fuse_integrate ? a.snap : a.parts);
+  T* me = static_cast<T*>(a.parts) + size_t(s) * kPartFields;
   const T steps = T(a.sub_steps == 0 ? 1 : a.sub_steps);
   T Fs[3] = {me[PF_FORCE], me[PF_FORCE + 1], me[PF_FORCE + 2]};
   T Ts = me[PF_TORQUE];
@@ -695,17 +701,16 @@ __global__ __launch_bounds__(64) void k_contact(ParticleArgs a) {
       }
     }
   }
-  // only force/torque of this particle are written; no block reads them
+  // only this particle's record is written; other blocks read the snapshot
+  // (fused) or only positions, rotations, masses and radii (not fused)
   for (int k = 0; k < D; ++k) me[PF_FORCE + k] = Fs[k];
   if (D == 2) me[PF_TORQUE] = Ts;
+  if (a.fuse_integrate) integrate_one<T, D>(a, me);
 }
 
 // integrate_particle + verlet_step_lambda (integrate.hpp:45-83, particle.hpp:171-211)
 template <class T, int D>
-__global__ void k_integrate(ParticleArgs a) {
-  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= a.n) return;
-  T* P = static_cast<T*>(a.parts) + size_t(s) * kPartFields;
+__device__ void integrate_one(const ParticleArgs& a, T* P) {
   // FpLbmOneParticle moves only for sub_steps > 0 and sets only the linear
   // acceleration (fplbm.hpp:233-244)
   if (a.fplbm && a.sub_steps == 0) return;
@@ -741,6 +746,13 @@ __global__ void k_integrate(ParticleArgs a) {
       P[PF_ROT + k] = rn;
     }
   }
+}
+
+template <class T, int D>
+__global__ void k_integrate(ParticleArgs a) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.n) return;
+  integrate_one<T, D>(a, static_cast<T*>(a.parts) + size_t(s) * kPartFields);
 }
 
 // ---------------------------------------------------------------- launchers
