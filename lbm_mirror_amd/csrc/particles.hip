@@ -409,13 +409,25 @@ __global__ __launch_bounds__(256) void k_gather(ParticleArgs a) {
   }
 }
 
-// Particle p's gather partials summed in block order (FP64), then its force
-// (and torque where the coupling has one) set from them.
+// Particle p's gather partials summed in FP64 by one 64-lane wavefront, in an
+// order fixed by the block count: up to 16 blocks lane 0 adds them in block
+// order; above that lane l adds blocks l, l+64, ... and an xor butterfly
+// combines the lanes (every lane ends with the same bits).  Lane 0 then sets
+// the particle's force (and torque where the coupling has one).
 template <class T, int D>
-__device__ __forceinline__ void finalize_one(const ParticleArgs& a, uint32_t p) {
+__device__ __forceinline__ void finalize_wave(const ParticleArgs& a, uint32_t p, int lane) {
   double s[6] = {0, 0, 0, 0, 0, 0};
-  for (uint32_t k = 0; k < a.blocks_per_particle; ++k)
-    for (int j = 0; j < 6; ++j) s[j] += a.partial[(size_t(p) * a.blocks_per_particle + k) * 6 + j];
+  const double* part = a.partial + size_t(p) * a.blocks_per_particle * 6;
+  if (a.blocks_per_particle <= 16) {
+    if (lane != 0) return;
+    for (uint32_t k = 0; k < a.blocks_per_particle; ++k)
+      for (int j = 0; j < 6; ++j) s[j] += part[k * 6 + j];
+  } else {
+    for (uint32_t k = uint32_t(lane); k < a.blocks_per_particle; k += 64)
+      for (int j = 0; j < 6; ++j) s[j] += part[k * 6 + j];
+    for (int j = 0; j < 6; ++j) s[j] = wave_sum(s[j]);
+    if (lane != 0) return;
+  }
   T* P = static_cast<T*>(a.parts) + size_t(p) * kPartFields;
   for (int d = 0; d < 3; ++d) {
     a.f64[p * 3 + d] = s[d];
@@ -429,11 +441,11 @@ __device__ __forceinline__ void finalize_one(const ParticleArgs& a, uint32_t p) 
   }
 }
 
+// one wavefront per particle
 template <class T, int D>
-__global__ void k_finalize(ParticleArgs a) {
-  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= a.n) return;
-  finalize_one<T, D>(a, p);
+__global__ __launch_bounds__(64) void k_finalize(ParticleArgs a) {
+  if (blockIdx.x >= a.n) return;  // uniform per block
+  finalize_wave<T, D>(a, blockIdx.x, int(threadIdx.x));
 }
 
 // build_cell_list (cell_list.hpp:81-108): bins filled in particle-index order
@@ -619,7 +631,7 @@ __global__ __launch_bounds__(64) void k_contact(ParticleArgs a) {
   // inside a full step the gather's finalize of this particle runs here (the
   // contact of s reads no other particle's force): one launch fewer per step
   if (a.fuse_finalize) {
-    if (lane == 0) finalize_one<T, D>(a, s);
+    finalize_wave<T, D>(a, s, lane);
     __syncthreads();
   }
   // fused with the integrate, this block moves its particle while others
@@ -769,7 +781,7 @@ static hipError_t couple_t(const ParticleArgs& a, hipStream_t s) {
   if (a.owner) hipLaunchKernelGGL((k_owner<L, T>), pg, dim3(256), 0, s, a);
   hipLaunchKernelGGL((k_gather<L, S, T, F>), pg, dim3(256), 0, s, a);
   if (!a.fuse_finalize)
-    hipLaunchKernelGGL((k_finalize<T, L::D>), dim3((a.n + 255) / 256), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((k_finalize<T, L::D>), dim3(a.n), dim3(64), 0, s, a);
   if (!scatter) {
     uint64_t bins = a.grid[0] * a.grid[1] * a.grid[2];
     hipLaunchKernelGGL((k_build_cells<T, L::D>), dim3(1), dim3(1024), 0, s, a, bins);
