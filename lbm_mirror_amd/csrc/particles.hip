@@ -77,6 +77,27 @@ __device__ inline T pop_at(const ParticleArgs& a, int i, int x, int y, int z) {
   return ldp<S, T>(f + size_t(L::opp[i]) * a.Np + c, T(L::w[i]));
 }
 
+// pop_at for a cell that exists when ok (else 0), with the load issued
+// whatever the case (an invalid neighbour reads the cell x, y, z itself, which
+// lies in the lattice): no branch around the load, so a caller's loads for
+// all directions can be in flight together.
+template <class L, class S, class T>
+__device__ __forceinline__ T pop_sel(const ParticleArgs& a, int i, int x, int y, int z, bool ok) {
+  const S* f = static_cast<const S*>(a.f);
+  const uint32_t nx = uint32_t(a.nx), ny = uint32_t(a.ny);
+  uint32_t c = uint32_t(x) + nx * (uint32_t(y) + ny * uint32_t(z));
+  int plane = i;
+  if (a.phase != 0) {
+    int xs = x - L::cx[i], ys = y - L::cy[i], zs = z - L::cz[i];
+    ok = ok && wrap(xs, a.nx, a.periodic & 1u) && wrap(ys, a.ny, a.periodic & 2u) &&
+         wrap(zs, a.nz, a.periodic & 4u);
+    if (ok) c = uint32_t(xs) + nx * (uint32_t(ys) + ny * uint32_t(zs));
+    plane = L::opp[i];
+  }
+  const T v = ldp<S, T>(f + size_t(plane) * a.Np + c, T(L::w[i]));
+  return ok ? v : T(0);
+}
+
 template <class T, int D>
 struct Box {
   int lo[3], hi[3];
@@ -205,10 +226,18 @@ __global__ __launch_bounds__(256) void k_owner(ParticleArgs a) {
   }
 }
 
-__device__ inline double wave_sum(double v) {
+// Six FP64 sums over the 64-lane wavefront by one xor butterfly: each step
+// exchanges all six values before adding, so the six chains share their
+// shuffle round trips.  Every lane ends with the same bits (a + b = b + a).
+__device__ inline void wave_sum6(double (&v)[6]) {
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+  for (int o = 32; o > 0; o >>= 1) {
+    double t[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) t[k] = __shfl_xor(v[k], o, 64);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) v[k] += t[k];
+  }
 }
 
 // Ownership without the per-cell owner array (a.owner == nullptr, at most
@@ -338,9 +367,12 @@ __global__ __launch_bounds__(256) void k_gather(ParticleArgs a) {
       // populations (FpLbmReset's compute_rho_momentum), else the stored macro
       T mom[3] = {T(0), T(0), T(0)};
       if (a.cflags[c] == 2) {
+        T fv[Q];
+#pragma unroll
+        for (int j = 1; j < Q; ++j) fv[j] = pop_sel<L, S, T>(a, j, x[0], x[1], x[2], true);
 #pragma unroll
         for (int j = 1; j < Q; ++j) {
-          const T fj = pop_at<L, S, T>(a, j, x[0], x[1], x[2]);
+          const T fj = fv[j];
 #pragma unroll
           for (int d = 0; d < D; ++d) {
             if (cdir<L>(j, d) == 1) mom[d] = mom[d] + fj;
@@ -360,19 +392,23 @@ __global__ __launch_bounds__(256) void k_gather(ParticleArgs a) {
       continue;
     }
     if (B >= T(1.0)) continue;
+    // every f_i at x and f_opp(i) at x + c_i first, all loads in flight at once
+    T fi[Q], fo[Q];
+#pragma unroll
+    for (int i = 1; i < Q; ++i) {
+      fi[i] = pop_sel<L, S, T>(a, i, x[0], x[1], x[2], true);
+      int xn = x[0] + L::cx[i], yn = x[1] + L::cy[i], zn = x[2] + L::cz[i];
+      const bool ok = wrap(xn, a.nx, a.periodic & 1u) && wrap(yn, a.ny, a.periodic & 2u) &&
+                      wrap(zn, a.nz, a.periodic & 4u);
+      fo[i] = pop_sel<L, S, T>(a, L::opp[i], ok ? xn : x[0], ok ? yn : x[1], ok ? zn : x[2], ok);
+    }
     T mom[3] = {T(0.0), T(0.0), T(0.0)};
 #pragma unroll
     for (int i = 1; i < Q; ++i) {
       T upe = T(0);
 #pragma unroll
       for (int d = 0; d < D; ++d) upe = upe + T(cdir<L>(i, d)) * vp[d];
-      const T fi = pop_at<L, S, T>(a, i, x[0], x[1], x[2]);
-      int xn = x[0] + L::cx[i], yn = x[1] + L::cy[i], zn = x[2] + L::cz[i];
-      T fo = T(0);
-      if (wrap(xn, a.nx, a.periodic & 1u) && wrap(yn, a.ny, a.periodic & 2u) &&
-          wrap(zn, a.nz, a.periodic & 4u))
-        fo = pop_at<L, S, T>(a, L::opp[i], xn, yn, zn);
-      const T added = fi * (T(1.0) - upe) + fo * (T(1.0) + upe);
+      const T added = fi[i] * (T(1.0) - upe) + fo[i] * (T(1.0) + upe);
 #pragma unroll
       for (int d = 0; d < D; ++d) mom[d] = mom[d] + T(cdir<L>(i, d)) * added;
     }
@@ -395,11 +431,9 @@ __global__ __launch_bounds__(256) void k_gather(ParticleArgs a) {
   // of the block in order through LDS
   __shared__ double red[4][6];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-#pragma unroll
-  for (int k = 0; k < 6; ++k) {
-    const double s = wave_sum(acc[k]);
-    if (lane == 0) red[wv][k] = s;
-  }
+  wave_sum6(acc);
+  if (lane == 0)
+    for (int k = 0; k < 6; ++k) red[wv][k] = acc[k];
   __syncthreads();
   if (threadIdx.x < 6) {
     double s = 0.0;
@@ -425,7 +459,7 @@ __device__ __forceinline__ void finalize_wave(const ParticleArgs& a, uint32_t p,
   } else {
     for (uint32_t k = uint32_t(lane); k < a.blocks_per_particle; k += 64)
       for (int j = 0; j < 6; ++j) s[j] += part[k * 6 + j];
-    for (int j = 0; j < 6; ++j) s[j] = wave_sum(s[j]);
+    wave_sum6(s);
     if (lane != 0) return;
   }
   T* P = static_cast<T*>(a.parts) + size_t(p) * kPartFields;
@@ -525,6 +559,9 @@ __global__ __launch_bounds__(256) void k_product(ParticleArgs a) {
 }
 
 // ---------------------------------------------------------------- DEM
+template <class T, int D>
+__device__ void integrate_one(const ParticleArgs& a, T* P);
+
 template <class T, int D>
 struct Man {
   T depth, n[3], pt[3];
