@@ -211,16 +211,21 @@ def test_against_reference_build_fresh_cases(tmp_path):
         assert _match(_py_args(argv), want), (argv, _py_args(argv), want)
 
 
-def test_vawt_example_rejects_bad_arguments(tmp_path):
-    """The rotor example's command line goes through the argument parser
-    before any device call: a negative step count or an unknown name ends the
-    program with status 1 and a message, without touching a GPU."""
+@pytest.mark.parametrize("example,bad", [
+    ("vawt_3d", (["--time_steps", "-3"], ["--axis_q", "1"], ["--stable_faces", "maybe"])),
+    ("moving_particles_2d_psm", (["--sub_steps", "-1"], ["--radius", "2"], ["--vtk_every", ""])),
+])
+def test_examples_reject_bad_arguments(tmp_path, example, bad):
+    """The example drivers' command lines go through the argument parser
+    before any device call: a negative count, an unknown name or an empty
+    number ends the program with status 1 and a message, without touching a
+    GPU or creating the output directory."""
     import subprocess
-    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "lbm_mirror_amd", "_build", "vawt_3d")
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "lbm_mirror_amd", "_build", example)
     if not os.path.exists(exe):
-        pytest.skip("vawt_3d example not built")
+        pytest.skip(f"{example} example not built")
     env = dict(os.environ, HOME=str(tmp_path))
-    for argv in (["--time_steps", "-3"], ["--axis_q", "1"], ["--stable_faces", "maybe"]):
+    for argv in bad:
         r = subprocess.run([exe, *argv], capture_output=True, text=True, timeout=60, env=env)
         assert r.returncode == 1 and r.stderr, argv
     assert not os.path.exists(tmp_path / ".lbm")
