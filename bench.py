@@ -464,7 +464,7 @@ def bench_sphere3d(args):
     from lbm_mirror_amd.warmstart import init_equilibrium_field
     from helpers import make_particle
 
-    def run(ext, steps):
+    def run(ext, steps, warmup):
         nx, ny, nz = ext
         lat = Lattice(abi.D3Q27, abi.F32, ext)
         f = np.ones((nz, ny, nx), np.uint8)
@@ -497,22 +497,24 @@ def bench_sphere3d(args):
             prm = abi.particle_params(k_n=1.0e-2, zeta_n=0.3, walls=True, sub_steps=16, max_radius=r,
                                       coupling=abi.COUPLING_HLBM)
         lat.particles_set([p], prm)
-        lat.step(args.warmup, t0=0)
+        lat.step(warmup, t0=0)
         lat.sync()
-        lat.step(steps, t0=args.warmup)
+        lat.step(steps, t0=warmup)
         lat.sync()
         ms = lat.last_step_ms()
         q = lat.particles_get()[0]
         umax = float(np.max(np.abs(lat.get_macros()[1])))
         lat.close()
         cells = nx * ny * nz
-        return {"extent": list(ext), "steps": steps, "ms_per_step": round(ms / steps, 5),
+        return {"extent": list(ext), "steps": steps, "warmup": warmup, "ms_per_step": round(ms / steps, 5),
                 "MLUPS": round(cells * steps / (ms * 1e3), 1),
                 "position": [round(x, 5) for x in q.position], "rotation_y": q.rotation[1],
                 "max_abs_u": umax}
 
-    small = run((32, 32, 32), 256)
-    big = run((256, 128, 128), args.steps if args.steps != 1000 else 400)
+    # the 32^3 lattice replays 32-step graphs: a warm-up of 65 steps holds
+    # its one capture, so the timed steps are replays
+    small = run((32, 32, 32), 256, max(args.warmup, 65))
+    big = run((256, 128, 128), args.steps if args.steps != 1000 else 400, args.warmup)
     name = "rotor (ParticleAxisIntegrate)" if args.spin else "free sphere with wall contact"
     return {"metric": f"MLUPS (D3Q27 FP32 HLBM {name}, full step incl. particle)",
             "value": big["MLUPS"], "unit": "MLUPS", "n_gpus": 1, "steps": big["steps"],

@@ -27,6 +27,7 @@ struct StepArgs {
   uint32_t N, Np;
   uint32_t periodic;
   uint64_t t;
+  const uint64_t* t_dev;  // null: t is the step index; else the index is *t_dev + t
   int32_t write_macros;
 };
 
@@ -37,6 +38,10 @@ struct LaunchShape {
 
 hipError_t launch_aa_step(int lattice, int storage, bool odd, const StepArgs& a,
                           const LaunchShape& shape, hipStream_t s);
+
+// *p = v and *p += v, in stream order (a step index that captured launches read)
+hipError_t launch_set_u64(uint64_t* p, uint64_t v, hipStream_t s);
+hipError_t launch_add_u64(uint64_t* p, uint64_t v, hipStream_t s);
 
 // ---- batched ensembles -------------------------------------------------------
 // A tile is `rows` consecutive rows (from y) of one member, blockDim cells wide

@@ -88,6 +88,8 @@ struct lbm_ctx {
   // of the launch arguments they captured (0 steps: every step launched)
   uint32_t steps_per_graph = 0;
   hipGraphExec_t step_graph = nullptr;
+  uint64_t* t_dev = nullptr;  // the replayed graph's first step index
+  uint64_t t_dev_value = ~uint64_t(0);  // what *t_dev holds once queued work ran (~0: unknown)
   std::vector<uint8_t> graph_key;
   // particles
   uint32_t np = 0;
@@ -290,7 +292,9 @@ lbm_status particle_pass(lbm_ctx* c, uint64_t t) {
   return LBM_OK;
 }
 
-StepArgs step_args(const lbm_ctx* c, uint64_t t, bool write_macros) {
+// t_dev: where captured launches find the step index of the first step they
+// replay (t is then the offset from it), else null
+StepArgs step_args(const lbm_ctx* c, uint64_t t, bool write_macros, const uint64_t* t_dev = nullptr) {
   StepArgs a;
   std::memset(&a, 0, sizeof a);  // padding too: the bytes key the step graphs
   a.f = c->f;
@@ -310,28 +314,17 @@ StepArgs step_args(const lbm_ctx* c, uint64_t t, bool write_macros) {
   a.Np = c->Np;
   a.periodic = c->periodic;
   a.t = t;
+  a.t_dev = t_dev;
   a.write_macros = write_macros ? 1 : 0;
   return a;
 }
 
-lbm_status fluid_pass(lbm_ctx* c, uint64_t t, bool write_macros) {
-  const StepArgs a = step_args(c, t, write_macros);
+lbm_status fluid_pass(lbm_ctx* c, uint64_t t, bool write_macros, const uint64_t* t_dev = nullptr) {
+  const StepArgs a = step_args(c, t, write_macros, t_dev);
   HIP_TRY(launch_aa_step(c->lattice, c->storage, c->phase == 1, a, c->shape, c->stream));
   c->phase ^= 1;
   return LBM_OK;
 }
-
-// The step index enters the kernels only through the Zou/He density ramps;
-// from the first step past every ramp on, all steps launch identical work.
-uint64_t ramp_end(const lbm_ctx* c) {
-  uint64_t e = 0;
-  for (int i = 0; i < kMaxOps; ++i)
-    if (c->has_op[i]) e = std::max<uint64_t>(e, c->ops_h[i].ramp_steps);
-  return e;
-}
-
-// Any step index past every ramp: what the captured kernels are given.
-constexpr uint64_t kGraphStep = uint64_t(1) << 62;
 
 template <class X>
 void append_bytes(std::vector<uint8_t>& k, const X& x) {
@@ -339,12 +332,18 @@ void append_bytes(std::vector<uint8_t>& k, const X& x) {
   k.insert(k.end(), p, p + sizeof(X));
 }
 
-// steps_per_graph full steps (particle pass and fluid pass, from phase 0)
-// launched as one graph, captured again whenever anything its launches
-// captured has changed
-lbm_status replay_steps(lbm_ctx* c, bool every) {
+// steps_per_graph full steps from step t (particle pass and fluid pass, from
+// phase 0) launched as one graph, captured again whenever anything its
+// launches captured has changed.  The step index enters the kernels only
+// through the Zou/He density ramps: the captured fluid launches read it as
+// *t_dev + their offset in the graph, and the graph's last node advances
+// *t_dev by its length, so one graph serves every start step, inside the
+// ramps or past them; *t_dev is set only where a replay does not continue
+// the previous one.
+lbm_status replay_steps(lbm_ctx* c, uint64_t t, bool every) {
+  if (!c->t_dev) HIP_TRY(hipMalloc((void**)&c->t_dev, sizeof(uint64_t)));
   std::vector<uint8_t> key;
-  append_bytes(key, step_args(c, kGraphStep, every));
+  append_bytes(key, step_args(c, 0, every, c->t_dev));
   append_bytes(key, c->shape);
   append_bytes(key, c->steps_per_graph);
   append_bytes(key, c->np);
@@ -360,9 +359,11 @@ lbm_status replay_steps(lbm_ctx* c, bool every) {
     HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     lbm_status s = LBM_OK;
     for (uint32_t i = 0; i < c->steps_per_graph && !s; ++i) {
-      if (c->np > 0) s = particle_pass(c, kGraphStep);
-      if (!s) s = fluid_pass(c, kGraphStep, every);
+      if (c->np > 0) s = particle_pass(c, i);
+      if (!s) s = fluid_pass(c, i, every, c->t_dev);
     }
+    if (!s && launch_add_u64(c->t_dev, c->steps_per_graph, c->stream) != hipSuccess)
+      s = fail(LBM_ERR_CRITICAL, "step index advance");
     hipError_t e = hipStreamEndCapture(c->stream, &g);
     if (s) return s;
     HIP_TRY(e);
@@ -370,7 +371,9 @@ lbm_status replay_steps(lbm_ctx* c, bool every) {
     HIP_TRY(hipGraphDestroy(g));
     c->graph_key = key;  // capture flipped the phase an even number of times
   }
+  if (c->t_dev_value != t) HIP_TRY(launch_set_u64(c->t_dev, t, c->stream));
   HIP_TRY(hipGraphLaunch(c->step_graph, c->stream));
+  c->t_dev_value = t + c->steps_per_graph;
   return LBM_OK;
 }
 
@@ -454,7 +457,7 @@ lbm_status lbm_ctx_destroy(lbm_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   void* ptrs[] = {c->f, c->flags, c->ops, c->rho, c->u, c->force, c->por, c->bcvel, c->pn, c->pd,
                   c->parts, c->snap, c->f64, c->t64, c->partial, c->cl_count, c->cl_slot,
-                  c->cl_overflow, c->owner};
+                  c->cl_overflow, c->owner, c->t_dev};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->step_graph) (void)hipGraphExecDestroy(c->step_graph);
@@ -733,15 +736,14 @@ lbm_status lbm_step(lbm_ctx* c, uint64_t t0, uint64_t nsteps) {
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
   // FPLBM reads the stored macro velocity of non-fluid cells every step
   const bool every = c->np > 0 && c->pp.coupling == LBM_COUPLING_FPLBM;
-  const uint64_t ramps_done = ramp_end(c);
   const uint64_t G = c->steps_per_graph;
   uint64_t k = 0;
   while (k < nsteps) {
     const uint64_t t = t0 + k;
-    // a whole graph of steps: past every ramp, from phase 0, and never the
-    // last step of the call (it writes the macros)
-    if (G > 0 && t >= ramps_done && c->phase == 0 && nsteps - k > G) {
-      s = replay_steps(c, every);
+    // a whole graph of steps: from phase 0, and never the last step of the
+    // call (it writes the macros)
+    if (G > 0 && c->phase == 0 && nsteps - k > G) {
+      s = replay_steps(c, t, every);
       if (s) return s;
       k += G;
       continue;
@@ -1015,7 +1017,6 @@ struct lbm_batch {
   double* d_partial = nullptr;
   double* d_res = nullptr;
   uint64_t* d_t0 = nullptr;
-  uint64_t* h_t0 = nullptr;  // pinned staging for the chain's first step index
   std::vector<void*> uprev;
   uint32_t ntiles = 0;
   size_t tiles_cap = 0;
@@ -1129,8 +1130,7 @@ lbm_status run_chain(lbm_batch* b, uint64_t t0, uint64_t n, bool with_res) {
     lbm_status s = build_tiles(b);
     if (s) return s;
   }
-  *b->h_t0 = t0;
-  HIP_TRY(hipMemcpyAsync(b->d_t0, b->h_t0, sizeof(uint64_t), hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(launch_set_u64(b->d_t0, t0, b->stream));
   const bool graphable = (n % 2 == 0) && b->phase == 0;
   if (!graphable) {
     lbm_status s = enqueue_chain(b, n, with_res);
@@ -1204,7 +1204,6 @@ lbm_status lbm_batch_create(lbm_batch** out, int32_t lattice, int32_t storage, u
   if (e == hipSuccess) e = hipMalloc((void**)&b->d_uprev, count * sizeof(void*));
   if (e == hipSuccess) e = hipMalloc((void**)&b->d_res, 2 * count * sizeof(double));
   if (e == hipSuccess) e = hipMalloc((void**)&b->d_t0, sizeof(uint64_t));
-  if (e == hipSuccess) e = hipHostMalloc((void**)&b->h_t0, sizeof(uint64_t), hipHostMallocDefault);
   if (e == hipSuccess)
     e = hipMemcpy(b->d_uprev, b->uprev.data(), count * sizeof(void*), hipMemcpyHostToDevice);
   if (e != hipSuccess) return bail(fail(LBM_ERR_OUT_OF_MEMORY, "batch arrays: %s", hipGetErrorString(e)));
@@ -1222,7 +1221,6 @@ lbm_status lbm_batch_destroy(lbm_batch* b) {
   void* ptrs[] = {b->d_members, b->d_tiles, b->d_tile_begin, b->d_uprev, b->d_partial, b->d_res, b->d_t0};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
-  if (b->h_t0) (void)hipHostFree(b->h_t0);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
   if (b->stream) (void)hipStreamDestroy(b->stream);

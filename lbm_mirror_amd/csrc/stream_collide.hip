@@ -139,8 +139,9 @@ __device__ __forceinline__ void apply_op(T (&f)[L::Q], OpP op, const StepArgs& a
         break;
       }
       case LBM_BC_ZOUHE_PRESSURE_X: {
-        const double rs = (op->ramp_steps > 0 && a.t < op->ramp_steps)
-                              ? op->rho_start + op->rho_slope * double(a.t)
+        const uint64_t t = a.t_dev ? *a.t_dev + a.t : a.t;
+        const double rs = (op->ramp_steps > 0 && t < op->ramp_steps)
+                              ? op->rho_start + op->rho_slope * double(t)
                               : op->rho_final;
         if constexpr (Q == 9) zouhe_p_d2q9<T>(f, op->east != 0, T(rs));
         if constexpr (Q == 27)
@@ -807,6 +808,7 @@ __global__ __launch_bounds__(256) void k_aa_batch(BatchArgs b) {
     const Tile tl = b.tiles[tile];
     StepArgs a = b.members[tl.member];
     a.t = t;
+    a.t_dev = nullptr;
     a.write_macros = b.write_macros != 0;
     if (b.write_macros == 2) a.u = b.u_prev[tl.member];
 
@@ -956,16 +958,36 @@ static hipError_t launch_l(int storage, bool odd, const StepArgs& a, const Launc
                            hipStream_t s) {
   switch (storage) {
     case LBM_F64:
-      if (sh.vec && kVecF64 && a.nx % 2 == 0) return launch_vec<L, double, true, 2>(odd, a, s);
+      if constexpr (kVecF64)
+        if (sh.vec && a.nx % 2 == 0) return launch_vec<L, double, true, 2>(odd, a, s);
       return launch_t<L, double, double, true>(odd, a, sh, s);
     case LBM_F32:
-      if (sh.vec && kVecF32 > 0 && (!odd || kVecOdd) && a.nx % kVecF32 == 0)
-        return launch_vec<L, float, true, (kVecF32 > 0 ? kVecF32 : 4)>(odd, a, s);
+      if constexpr (kVecF32 > 0)
+        if (sh.vec && (!odd || kVecOdd) && a.nx % kVecF32 == 0)
+          return launch_vec<L, float, true, (kVecF32 > 0 ? kVecF32 : 4)>(odd, a, s);
       return launch_t<L, float, float, true>(odd, a, sh, s);
     case LBM_F16S: return launch_t<L, __half, float, false>(odd, a, sh, s);
     case LBM_BF16S: return launch_t<L, __hip_bfloat16, float, false>(odd, a, sh, s);
   }
   return hipErrorInvalidValue;
+}
+
+__global__ void k_set_u64(uint64_t* p, uint64_t v) {
+  if (threadIdx.x == 0) *p = v;
+}
+
+hipError_t launch_set_u64(uint64_t* p, uint64_t v, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, s, p, v);
+  return hipGetLastError();
+}
+
+__global__ void k_add_u64(uint64_t* p, uint64_t v) {
+  if (threadIdx.x == 0) *p = *p + v;
+}
+
+hipError_t launch_add_u64(uint64_t* p, uint64_t v, hipStream_t s) {
+  hipLaunchKernelGGL(k_add_u64, dim3(1), dim3(64), 0, s, p, v);
+  return hipGetLastError();
 }
 
 hipError_t launch_aa_step(int lattice, int storage, bool odd, const StepArgs& a,

@@ -2,8 +2,9 @@
 launched one by one: bit-identical populations, macros and particle records
 after every call of a sequence of calls of uneven lengths, with host
 synchronisation and read-back between them, including calls that start
-inside a Zou/He density ramp, odd start phases and a change of an argument
-the graph captured (which forces a new capture)."""
+inside a Zou/He density ramp (the replayed launches read the step index
+from device memory, set before each replay), odd start phases and a change
+of an argument the graph captured (which forces a new capture)."""
 import numpy as np
 import pytest
 
@@ -13,14 +14,14 @@ from lbm_mirror_amd import abi
 pytestmark = pytest.mark.gpu
 
 
-def _c1(st, graph):
+def _c1(st, graph, ramp=40):
     from lbm_mirror_amd.lattice import Lattice
     nx, ny = 96, 24
     lat = Lattice(abi.D2Q9, st, (nx, ny))
     lat.set_flags(channel_flags_2d(nx, ny))
     ops = c1_ops()
     ops[4] = abi.op_desc(boundary=abi.BC_ZOUHE_PRESSURE_X, collision=abi.COLL_BGK, east=0, tau=0.8,
-                         ramp=(1.0, -0.002 / 40, 0.998, 40))
+                         ramp=(1.0, -0.002 / ramp, 0.998, ramp))
     for fl, op in ops.items():
         lat.set_operator(fl, op)
     lat.set_boundary_velocity(parabolic_inlet(nx, ny, 0.05))
@@ -64,9 +65,11 @@ def _same(a, b):
 CALLS = [1, 37, 100, 5, 64, 66, 3, 130]
 
 
+# ramp 400: every replay of the first calls runs inside the density ramp
+@pytest.mark.parametrize("ramp", [40, 400])
 @pytest.mark.parametrize("st", [abi.F64, abi.F32])
-def test_channel_graph_replays_match_launches(hip, st):
-    g, e = _c1(st, 32), _c1(st, 0)
+def test_channel_graph_replays_match_launches(hip, st, ramp):
+    g, e = _c1(st, 32, ramp), _c1(st, 0, ramp)
     t = 0
     for n in CALLS:
         g.step(n, t0=t)
