@@ -29,6 +29,9 @@
 
 namespace lbmx {
 
+// couplings, as gather template arguments
+constexpr int kCplPsm = 0, kCplHlbm = 1, kCplFplbm = 2;
+
 template <class S, class T>
 __device__ inline T ldp(const S* p, T w);
 template <>
@@ -317,10 +320,13 @@ struct LocalOwners {
   }
 };
 
-// S: population storage, T: particle arithmetic, F: fluid field type
-template <class L, class S, class T, class F>
+// S: population storage, T: particle arithmetic, F: fluid field type;
+// CPL: the coupling (kCplPsm, kCplHlbm, kCplFplbm), LOCAL: ownership decided
+// per block (no owner array)
+template <class L, class S, class T, class F, int CPL, bool LOCAL>
 __global__ __launch_bounds__(256) void k_gather(ParticleArgs a) {
   constexpr int D = L::D, Q = L::Q;
+  constexpr bool hlbm = CPL == kCplHlbm, fplbm = CPL == kCplFplbm;
   const uint32_t p = blockIdx.x / a.blocks_per_particle;
   const uint32_t chunk = blockIdx.x % a.blocks_per_particle;
   if (p >= a.n) return;  // uniform per block
@@ -332,20 +338,19 @@ __global__ __launch_bounds__(256) void k_gather(ParticleArgs a) {
   const T one = T(1.0);
   // ts = 1/sub_steps, u_p = (X − X_old)/ts (psm.hpp:160-172)
   // PSM divides by sub_steps as given (psm.hpp:160-172); HLBM takes 0 as 1
-  const T ts = one / T(a.hlbm && a.sub_steps == 0 ? uint64_t(1) : a.sub_steps);
+  const T ts = one / T(hlbm && a.sub_steps == 0 ? uint64_t(1) : a.sub_steps);
   // body velocity: (X - X_old)/ts, or (X - X_old)*sub_steps for FPLBM (fplbm.hpp:198-200)
   T vp[3] = {T(0), T(0), T(0)};
   for (int d = 0; d < D; ++d)
-    vp[d] = a.fplbm ? (P[PF_POS + d] - P[PF_POS_OLD + d]) * T(a.sub_steps) : (P[PF_POS + d] - P[PF_POS_OLD + d]) / ts;
+    vp[d] = fplbm ? (P[PF_POS + d] - P[PF_POS_OLD + d]) * T(a.sub_steps) : (P[PF_POS + d] - P[PF_POS_OLD + d]) / ts;
   F* por = static_cast<F*>(a.por);
   F* fout = static_cast<F*>(a.force);
   const bool scatter = a.porosity_mode == LBM_POROSITY_SCATTER;
-  const bool local = a.owner == nullptr;
   __shared__ int lo_cnt;
   __shared__ uint32_t lo_q[kOverlapCap];
   __shared__ int lo_box[kOverlapCap][6];
   const LocalOwners<T, D> lo{&lo_cnt, lo_q, lo_box};
-  if (local) lo.build(a, p, b);  // uniform per block
+  if constexpr (LOCAL) lo.build(a, p, b);
   double acc[6] = {0, 0, 0, 0, 0, 0};
   for (uint32_t k = chunk * blockDim.x + threadIdx.x; k < b.count;
        k += a.blocks_per_particle * blockDim.x) {
@@ -355,14 +360,14 @@ __global__ __launch_bounds__(256) void k_gather(ParticleArgs a) {
     const T B = porosity_at<T, D>(P, x, eps, rel);
     const uint32_t c = uint32_t(x[0]) + uint32_t(a.nx) * (uint32_t(x[1]) + uint32_t(a.ny) * uint32_t(x[2]));
     bool own_box, own_solid;
-    if (local) {
-      lo.test(a, p, x, !a.fplbm && B < T(1.0), eps, own_box, own_solid);
+    if constexpr (LOCAL) {
+      lo.test(a, p, x, !fplbm && B < T(1.0), eps, own_box, own_solid);
     } else {
       own_box = a.owner[2 * c] == p + 1;
-      own_solid = !a.fplbm && B < T(1.0) && a.owner[2 * c + 1] == p + 1;
+      own_solid = !fplbm && B < T(1.0) && a.owner[2 * c + 1] == p + 1;
     }
     if (scatter && own_box) por[c] = F(B);
-    if (a.fplbm) {
+    if constexpr (fplbm) {
       // FpLbmOneParticle (fplbm.hpp:204-224): m of a flag-2 cell from its
       // populations (FpLbmReset's compute_rho_momentum), else the stored macro
       T mom[3] = {T(0), T(0), T(0)};
@@ -413,7 +418,7 @@ __global__ __launch_bounds__(256) void k_gather(ParticleArgs a) {
       for (int d = 0; d < D; ++d) mom[d] = mom[d] + T(cdir<L>(i, d)) * added;
     }
     // PSM scales the exchange by the solid fraction; HLBM keeps it raw (hlbm.hpp:229-231)
-    const T flip = a.hlbm ? one : one - B;
+    const T flip = hlbm ? one : one - B;
     T fc[3] = {T(0), T(0), T(0)};
     for (int d = 0; d < D; ++d) fc[d] = mom[d] * flip;
     if (own_solid)
@@ -816,7 +821,16 @@ static hipError_t couple_t(const ParticleArgs& a, hipStream_t s) {
                      N, L::D, scatter ? 1 : 0);
   const dim3 pg(a.n * a.blocks_per_particle);
   if (a.owner) hipLaunchKernelGGL((k_owner<L, T>), pg, dim3(256), 0, s, a);
-  hipLaunchKernelGGL((k_gather<L, S, T, F>), pg, dim3(256), 0, s, a);
+  const int cpl = a.hlbm ? kCplHlbm : (a.fplbm ? kCplFplbm : kCplPsm);
+  if (a.owner) {
+    if (cpl == kCplHlbm) hipLaunchKernelGGL((k_gather<L, S, T, F, kCplHlbm, false>), pg, dim3(256), 0, s, a);
+    else if (cpl == kCplFplbm) hipLaunchKernelGGL((k_gather<L, S, T, F, kCplFplbm, false>), pg, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((k_gather<L, S, T, F, kCplPsm, false>), pg, dim3(256), 0, s, a);
+  } else {
+    if (cpl == kCplHlbm) hipLaunchKernelGGL((k_gather<L, S, T, F, kCplHlbm, true>), pg, dim3(256), 0, s, a);
+    else if (cpl == kCplFplbm) hipLaunchKernelGGL((k_gather<L, S, T, F, kCplFplbm, true>), pg, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((k_gather<L, S, T, F, kCplPsm, true>), pg, dim3(256), 0, s, a);
+  }
   if (!a.fuse_finalize)
     hipLaunchKernelGGL((k_finalize<T, L::D>), dim3(a.n), dim3(64), 0, s, a);
   if (!scatter) {
