@@ -192,6 +192,11 @@ lbm_status lbm_set_force_field(lbm_ctx* ctx, const double* force);
 lbm_status lbm_get_force_field(lbm_ctx* ctx, double* force);
 lbm_status lbm_set_porosity(lbm_ctx* ctx, const double* porosity);
 lbm_status lbm_get_porosity(lbm_ctx* ctx, double* porosity);
+/* HLBM particle sums as the last step left them: N, D doubles per cell
+ * (n[d*N + cell]), and D, one per cell — the reference's particle_N and
+ * particle_D fields (HlbmReset / HlbmOneParticleStamp, hlbm.hpp:28-49,264-308).
+ * Zero where no HLBM coupling has run. */
+lbm_status lbm_get_hlbm_sums(lbm_ctx* ctx, double* n, double* d);
 
 /* Populations: f[i*N + cell], the pre-collision (post-stream) populations of
  * the next step, i.e. what the reference holds in dfs_old (t even) or dfs

@@ -30,6 +30,7 @@ EXPORTS = [
     "lbm_last_error", "lbm_version", "lbm_device_count", "lbm_ctx_create", "lbm_ctx_destroy",
     "lbm_set_flags", "lbm_set_periodic", "lbm_set_operator", "lbm_set_boundary_velocity",
     "lbm_set_force_field", "lbm_get_force_field", "lbm_set_porosity", "lbm_get_porosity",
+    "lbm_get_hlbm_sums",
     "lbm_set_populations", "lbm_get_populations", "lbm_init_equilibrium", "lbm_get_macros",
     "lbm_set_macros", "lbm_step", "lbm_sync", "lbm_last_step_ms", "lbm_particles_set",
     "lbm_particles_get", "lbm_spheroid_mass", "lbm_psm_couple", "lbm_particle_reduce",
@@ -73,6 +74,7 @@ def load_library(path=None):
         getattr(L, name).argtypes = [vp, dp]
     for name in ("lbm_get_force_field", "lbm_get_porosity", "lbm_get_populations"):
         getattr(L, name).argtypes = [vp, dp]
+    L.lbm_get_hlbm_sums.argtypes = [vp, dp, dp]
     L.lbm_init_equilibrium.argtypes = [vp, C.c_double, dp]
     L.lbm_get_macros.argtypes = [vp, dp, dp]
     L.lbm_set_macros.argtypes = [vp, dp, dp]

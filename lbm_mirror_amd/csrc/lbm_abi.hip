@@ -605,6 +605,18 @@ lbm_status lbm_set_porosity(lbm_ctx* c, const double* p) {
   return put_field(c, c->por, p, c->N);
 }
 
+lbm_status lbm_get_hlbm_sums(lbm_ctx* c, double* n, double* d) {
+  if (!c || !n || !d) return fail(LBM_ERR_INVALID_ARG, "null argument");
+  if (!c->pn || !c->pd) {
+    std::fill(n, n + size_t(c->D) * c->N, 0.0);
+    std::fill(d, d + c->N, 0.0);
+    return LBM_OK;
+  }
+  lbm_status s = get_field(c, c->pn, n, size_t(c->D) * c->N);
+  if (s) return s;
+  return get_field(c, c->pd, d, c->N);
+}
+
 lbm_status lbm_get_porosity(lbm_ctx* c, double* p) {
   if (!c || !p) return fail(LBM_ERR_INVALID_ARG, "null argument");
   if (!c->por) {

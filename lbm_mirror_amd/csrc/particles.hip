@@ -180,31 +180,145 @@ __global__ void k_reset_cells(F* por, uint32_t* owner, F* pn, F* pd, F* force, u
 }
 
 
+// LDS entries of a block's list of overlapping particles
+constexpr int kOverlapCap = 256;
+
+template <class T, int D>
+__device__ __forceinline__ bool box_has(const Box<T, D>& q, const int* x) {
+  for (int d = 0; d < D; ++d)
+    if (x[d] < q.lo[d] || x[d] >= q.hi[d]) return false;
+  return true;
+}
+
 // HlbmOneParticleStamp (hlbm.hpp:264-308): over the post-move AABB,
 // D += 1 - B and N += u_p (1 - B) with B the porosity of this step's gather
-// and u_p = (X - X_old) / ts after the move.  Cells where two discs' solid
-// fractions overlap take both contributions by atomic addition.
+// and u_p = (X - X_old) / ts after the move.  The reference stamps particle
+// after particle, so a cell inside several post-move boxes sums their terms
+// in particle order.  Up to kLocalOwnerMax particles each block lists the
+// particles whose post-move boxes overlap its own (LDS, sorted by index; a
+// block whose list overflows tests them all); a cell is written only by the
+// lowest-index particle whose box holds it, which adds every holder's term in
+// index order to the reset value: the reference's sums, bit for bit, with
+// plain stores.  Above kLocalOwnerMax particles the terms are added
+// atomically (order not fixed where boxes overlap).
+template <class T, class F, int D>
+__device__ __forceinline__ void stamp_velocity(const T* P, T ts, T* vp) {
+  for (int d = 0; d < D; ++d) vp[d] = (P[PF_POS + d] - P[PF_POS_OLD + d]) / ts;
+}
+
 template <class T, class F, int D>
 __global__ __launch_bounds__(256) void k_hlbm_stamp(ParticleArgs a) {
   const uint32_t p = blockIdx.x / a.blocks_per_particle;
   const uint32_t chunk = blockIdx.x % a.blocks_per_particle;
-  if (p >= a.n) return;
-  const T* P = static_cast<const T*>(a.parts) + size_t(p) * kPartFields;
+  if (p >= a.n) return;  // uniform per block
+  const T* parts = static_cast<const T*>(a.parts);
+  const T* P = parts + size_t(p) * kPartFields;
   const Box<T, D> b = aabb_of<T, D>(P, a);
   const T ts = T(1.0) / T(a.sub_steps == 0 ? uint64_t(1) : a.sub_steps);
   T vp[3] = {T(0), T(0), T(0)};
-  for (int d = 0; d < D; ++d) vp[d] = (P[PF_POS + d] - P[PF_POS_OLD + d]) / ts;
+  stamp_velocity<T, F, D>(P, ts, vp);
   const F* por = static_cast<const F*>(a.por);
   F* pn = static_cast<F*>(a.pn);
   F* pd = static_cast<F*>(a.pd);
+  const bool ordered = a.n <= kLocalOwnerMax;  // uniform
+  __shared__ int cnt;
+  __shared__ uint32_t lq[kOverlapCap];
+  __shared__ int lbox[kOverlapCap][6];
+  if (ordered) {
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < a.n; j += blockDim.x) {
+      if (j == p) continue;
+      const Box<T, D> o = aabb_of<T, D>(parts + size_t(j) * kPartFields, a);
+      bool hit = o.count > 0;
+      for (int d = 0; d < D; ++d) hit = hit && o.lo[d] < b.hi[d] && b.lo[d] < o.hi[d];
+      if (!hit) continue;
+      const int e = atomicAdd(&cnt, 1);
+      if (e < kOverlapCap) {
+        lq[e] = j;
+        for (int d = 0; d < 3; ++d) {
+          lbox[e][d] = o.lo[d];
+          lbox[e][3 + d] = o.hi[d];
+        }
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {  // by particle index (the lists are short)
+      const int m = min(cnt, kOverlapCap);
+      for (int u = 1; u < m; ++u)
+        for (int v = u; v > 0 && lq[v - 1] > lq[v]; --v) {
+          const uint32_t tq = lq[v];
+          lq[v] = lq[v - 1];
+          lq[v - 1] = tq;
+          for (int d = 0; d < 6; ++d) {
+            const int tb = lbox[v][d];
+            lbox[v][d] = lbox[v - 1][d];
+            lbox[v - 1][d] = tb;
+          }
+        }
+    }
+    __syncthreads();
+  }
+  const int listed = ordered ? cnt : 0;
   for (uint32_t k = chunk * blockDim.x + threadIdx.x; k < b.count; k += a.blocks_per_particle * blockDim.x) {
     int x[3];
     box_cell<T, D>(b, k, x);
     const uint32_t c = uint32_t(x[0]) + uint32_t(a.nx) * (uint32_t(x[1]) + uint32_t(a.ny) * uint32_t(x[2]));
     const F flip = F(1.0) - por[c];
     if (flip == F(0)) continue;
-    atomicAdd(pd + c, flip);
-    for (int d = 0; d < D; ++d) atomicAdd(pn + size_t(d) * a.N + c, F(vp[d]) * flip);
+    if (!ordered) {
+      atomicAdd(pd + c, flip);
+      for (int d = 0; d < D; ++d) atomicAdd(pn + size_t(d) * a.N + c, F(vp[d]) * flip);
+      continue;
+    }
+    // the cell's holders in index order: p first unless a lower index holds it
+    F sd = pd[c];
+    F sn[3] = {F(0), F(0), F(0)};
+    for (int d = 0; d < D; ++d) sn[d] = pn[size_t(d) * a.N + c];
+    bool first = true, mine_added = false;
+    auto add = [&](const T* v) {
+      sd = sd + flip;
+      for (int d = 0; d < D; ++d) sn[d] = sn[d] + F(v[d]) * flip;
+    };
+    if (listed <= kOverlapCap) {
+      for (int e = 0; e < listed && first; ++e) {
+        bool in = true;
+        for (int d = 0; d < D; ++d) in = in && x[d] >= lbox[e][d] && x[d] < lbox[e][3 + d];
+        if (!in) continue;
+        if (lq[e] < p) {
+          first = false;
+          break;
+        }
+        if (!mine_added) {
+          add(vp);
+          mine_added = true;
+        }
+        T vq[3] = {T(0), T(0), T(0)};
+        stamp_velocity<T, F, D>(parts + size_t(lq[e]) * kPartFields, ts, vq);
+        add(vq);
+      }
+    } else {
+      for (uint32_t j = 0; j < a.n && first; ++j) {
+        if (j == p) {
+          add(vp);
+          mine_added = true;
+          continue;
+        }
+        const T* Pj = parts + size_t(j) * kPartFields;
+        if (!box_has<T, D>(aabb_of<T, D>(Pj, a), x)) continue;
+        if (j < p) {
+          first = false;
+          break;
+        }
+        T vq[3] = {T(0), T(0), T(0)};
+        stamp_velocity<T, F, D>(Pj, ts, vq);
+        add(vq);
+      }
+    }
+    if (!first) continue;
+    if (!mine_added) add(vp);
+    pd[c] = sd;
+    for (int d = 0; d < D; ++d) pn[size_t(d) * a.N + c] = sn[d];
   }
 }
 
@@ -249,15 +363,6 @@ __device__ inline void wave_sum6(double (&v)[6]) {
 // (B < 1) and no q > p is solid there -- what the atomic maxima of k_owner
 // decide.  Each gather block first lists the q > p whose boxes intersect p's
 // (LDS, kOverlapCap entries); a block whose list overflows tests every q > p.
-constexpr int kOverlapCap = 256;
-
-template <class T, int D>
-__device__ __forceinline__ bool box_has(const Box<T, D>& q, const int* x) {
-  for (int d = 0; d < D; ++d)
-    if (x[d] < q.lo[d] || x[d] >= q.hi[d]) return false;
-  return true;
-}
-
 template <class T, int D>
 struct LocalOwners {
   int* cnt;        // LDS: entries found (may exceed kOverlapCap)

@@ -94,6 +94,13 @@ class Lattice:
         check(self.L.lbm_get_porosity(self._h, _dp(a)))
         return a
 
+    def get_hlbm_sums(self):
+        """HLBM particle sums N (D x N) and D (N) after the last step."""
+        n = np.zeros(self.D * self.N)
+        d = np.zeros(self.N)
+        check(self.L.lbm_get_hlbm_sums(self._h, _dp(n), _dp(d)))
+        return n.reshape(self.D, self.N), d
+
     def set_populations(self, f):
         a = np.ascontiguousarray(f, dtype=np.float64).reshape(-1)
         if a.size != self.Q * self.N:
