@@ -9,7 +9,7 @@ import os
 import numpy as np
 import pytest
 
-from helpers import c1_ops, c2_ops, channel_flags_2d, duct_flags_3d, parabolic_inlet
+from helpers import c1_ops, c2_ops, channel_flags_2d, duct_flags_3d, make_particle, parabolic_inlet
 from lbm_mirror_amd import LbmError, abi, output
 
 pytestmark = pytest.mark.gpu
@@ -139,3 +139,63 @@ def test_sixteen_bit_populations_have_no_forst_layout(hip, tmp_path):
     output.write_forst(lat, str(tmp_path / "m.forst"), "velocity,density")
     assert os.path.getsize(tmp_path / "m.forst") == lat.N * 3 * 4
     lat.close()
+
+
+def _particle_channel(st, coupling):
+    from lbm_mirror_amd.lattice import Lattice, spheroid_mass
+    from helpers import seeded_discs
+    nx, ny = 160, 64
+    lat = Lattice(abi.D2Q9, st, (nx, ny))
+    lat.set_flags(channel_flags_2d(nx, ny))
+    coll = {abi.COUPLING_PSM: abi.COLL_PSM, abi.COUPLING_HLBM: abi.COLL_HLBM,
+            abi.COUPLING_FPLBM: abi.COLL_FPLBM}[coupling]
+    ops = {1: abi.op_desc(boundary=abi.BC_BOUNCE_BACK), 2: abi.op_desc(collision=coll, tau=0.7),
+           3: abi.op_desc(boundary=abi.BC_ZOUHE_PRESSURE_X, collision=abi.COLL_BGK, east=1, tau=0.7,
+                          ramp=(1.0, 0.002 / 30, 1.002, 30)),
+           4: abi.op_desc(boundary=abi.BC_ZOUHE_PRESSURE_X, collision=abi.COLL_BGK, east=0, tau=0.7, rho=1.0)}
+    for fl, op in ops.items():
+        lat.set_operator(fl, op)
+    parts = []
+    for k, (x, y, r) in enumerate(seeded_discs(6, (nx, ny), 4.0, 7.0, 9, margin=4.0, gap=2.0)):
+        m, i = spheroid_mass(2, st, r, 1.2)
+        parts.append(make_particle([x, y], r, 1.2, 2, st, vel=[0.2 / 8 * (-1) ** k, 0.05 / 8], mass=m, inertia=i))
+    prm = abi.particle_params(k_n=1e2, zeta_n=0.3, walls=True, sub_steps=8, body_acceleration=(0.0, -1e-4),
+                              max_radius=7.0, coupling=coupling)
+    return lat, parts, prm
+
+
+@pytest.mark.parametrize("coupling", [abi.COUPLING_PSM, abi.COUPLING_HLBM, abi.COUPLING_FPLBM])
+def test_particle_run_resumes_from_checkpoint_bit_for_bit(hip, tmp_path, coupling):
+    """A particle-laden run checkpointed mid-way (Forst: info, dfs, velocity,
+    density, force, porosity; the particle records through lbm_particles_get)
+    and resumed in a new context (lbm_read_forst, lbm_particles_set) continues
+    bit for bit like the run that never stopped, across the inlet ramp."""
+    st = abi.F32
+    t1, t2 = 23, 41
+    members = "info,dfs,velocity,density,force,porosity"
+    straight, parts, prm = _particle_channel(st, coupling)
+    straight.init_equilibrium(1.0, (0.0, 0.0, 0.0))
+    straight.particles_set(parts, prm)
+    straight.step(t1 + t2, t0=0)
+    first, parts, prm = _particle_channel(st, coupling)
+    first.init_equilibrium(1.0, (0.0, 0.0, 0.0))
+    first.particles_set(parts, prm)
+    first.step(t1, t0=0)
+    path = str(tmp_path / "ckpt.forst")
+    output.write_forst(first, path, members)
+    records = first.particles_get()
+    first.close()
+    resumed, _, prm = _particle_channel(st, coupling)
+    output.read_forst(resumed, path, members)
+    resumed.particles_set(records, prm)
+    resumed.step(t2, t0=t1)
+    assert np.array_equal(resumed.get_populations(), straight.get_populations())
+    for a, b in zip(resumed.get_macros(), straight.get_macros()):
+        assert np.array_equal(a, b)
+    assert np.array_equal(resumed.get_porosity(), straight.get_porosity())
+    assert np.array_equal(resumed.get_force_field(), straight.get_force_field())
+    for a, b in zip(resumed.particles_get(), straight.particles_get()):
+        for fld in ("position", "position_old", "rotation", "rotation_old", "acceleration", "force", "torque"):
+            assert list(getattr(a, fld)) == list(getattr(b, fld)), fld
+    resumed.close()
+    straight.close()
