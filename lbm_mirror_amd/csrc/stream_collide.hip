@@ -814,8 +814,14 @@ static hipError_t launch_vec(bool odd, const StepArgs& a, hipStream_t s) {
 // except in the member's last tile; each cell runs the single-lattice body.
 // The step index is read from device memory (t0 + k) so a chain of steps can
 // be captured once in a hipGraph and replayed.
+// D2Q9 batch: 6 waves per SIMD (≤ 80 VGPRs; it takes 90 unconstrained, 5
+// waves): 67.4 -> 69.7 GLUPS on C4; 5 waves 67.1, 7 waves 63.3, 8 spill (50.3)
+#ifndef LBMX_BATCH_WAVES
+#define LBMX_BATCH_WAVES 6
+#endif
 template <class L, class S, class T, bool WB, bool ODD>
-__global__ __launch_bounds__(256) void k_aa_batch(BatchArgs b) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L::Q == 9 ? LBMX_BATCH_WAVES : 1))) void k_aa_batch(
+    BatchArgs b) {
   const uint64_t t = *b.t0 + b.k;
   for (uint32_t tile = blockIdx.x; tile < b.ntiles; tile += gridDim.x) {
     const Tile tl = b.tiles[tile];
