@@ -789,25 +789,33 @@ __global__ __launch_bounds__(64) void k_contact(ParticleArgs a) {
   T Fs[3] = {me[PF_FORCE], me[PF_FORCE + 1], me[PF_FORCE + 2]};
   T Ts = me[PF_TORQUE];
   const T rl = me[PF_RADIUS];
-  for (uint32_t j0 = 0; j0 < a.n; j0 += 64) {
-    const uint32_t jl = j0 + uint32_t(lane);
-    bool cand = false;
-    if (jl < a.n && jl != s) {
-      // broadphase_collision_check (particle.hpp:247-279)
-      const T* ot = parts + size_t(jl) * kPartFields;
-      T n2 = T(0);
-      for (int k = 0; k < D; ++k) {
-        const T dd = me[PF_POS + k] - ot[PF_POS + k];
-        n2 = n2 + dd * dd;
+  // four chunks of 64 partners per round: their broadphase loads are in
+  // flight together, then lane 0 takes the candidates in ascending order
+  constexpr int kChunks = 4;
+  for (uint32_t g0 = 0; g0 < a.n; g0 += 64 * kChunks) {
+    uint64_t masks[kChunks];
+#pragma unroll
+    for (int q = 0; q < kChunks; ++q) {
+      const uint32_t jl = g0 + 64u * q + uint32_t(lane);
+      bool cand = false;
+      if (jl < a.n && jl != s) {
+        // broadphase_collision_check (particle.hpp:247-279)
+        const T* ot = parts + size_t(jl) * kPartFields;
+        T n2 = T(0);
+        for (int k = 0; k < D; ++k) {
+          const T dd = me[PF_POS + k] - ot[PF_POS + k];
+          n2 = n2 + dd * dd;
+        }
+        const T rr = ot[PF_RADIUS];
+        const T rab2 = (rl * rl + rr * rr) + (rr * rl) * T(2.0);
+        cand = n2 < rab2;
       }
-      const T rr = ot[PF_RADIUS];
-      const T rab2 = (rl * rl + rr * rr) + (rr * rl) * T(2.0);
-      cand = n2 < rab2;
+      masks[q] = __ballot(cand);
     }
-    uint64_t mask = __ballot(cand);
     if (lane != 0) continue;
-    while (mask) {
-      const uint32_t j = j0 + uint32_t(__ffsll((unsigned long long)mask) - 1);
+    for (int q = 0; q < kChunks; ++q)
+    for (uint64_t mask = masks[q]; mask;) {
+      const uint32_t j = g0 + 64u * q + uint32_t(__ffsll((unsigned long long)mask) - 1);
       mask &= mask - 1;
       const T* ot = parts + size_t(j) * kPartFields;
       const bool self_lo = s < j;
