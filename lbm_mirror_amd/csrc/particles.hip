@@ -64,26 +64,11 @@ __device__ inline bool wrap(int& v, int n, bool periodic) {
   return true;
 }
 
-// Pre-collision population f_i at (x,y,z) in the current AA phase.
-template <class L, class S, class T>
-__device__ inline T pop_at(const ParticleArgs& a, int i, int x, int y, int z) {
-  const S* f = static_cast<const S*>(a.f);
-  if (a.phase == 0) {
-    const uint32_t c = uint32_t(x) + uint32_t(a.nx) * (uint32_t(y) + uint32_t(a.ny) * uint32_t(z));
-    return ldp<S, T>(f + size_t(i) * a.Np + c, T(L::w[i]));
-  }
-  int xs = x - L::cx[i], ys = y - L::cy[i], zs = z - L::cz[i];
-  if (!wrap(xs, a.nx, a.periodic & 1u) || !wrap(ys, a.ny, a.periodic & 2u) ||
-      !wrap(zs, a.nz, a.periodic & 4u))
-    return T(0);
-  const uint32_t c = uint32_t(xs) + uint32_t(a.nx) * (uint32_t(ys) + uint32_t(a.ny) * uint32_t(zs));
-  return ldp<S, T>(f + size_t(L::opp[i]) * a.Np + c, T(L::w[i]));
-}
-
-// pop_at for a cell that exists when ok (else 0), with the load issued
-// whatever the case (an invalid neighbour reads the cell x, y, z itself, which
-// lies in the lattice): no branch around the load, so a caller's loads for
-// all directions can be in flight together.
+// Pre-collision population f_i at (x,y,z) in the current AA phase (phase 1:
+// plane opp(i) at x - c_i, 0 outside the lattice), or 0 when !ok.  The load is
+// issued whatever the case (an invalid neighbour reads the cell x, y, z
+// itself, which lies in the lattice): no branch around the load, so a caller's
+// loads for all directions can be in flight together.
 template <class L, class S, class T>
 __device__ __forceinline__ T pop_sel(const ParticleArgs& a, int i, int x, int y, int z, bool ok) {
   const S* f = static_cast<const S*>(a.f);
@@ -201,7 +186,7 @@ __device__ __forceinline__ bool box_has(const Box<T, D>& q, const int* x) {
 // index order to the reset value: the reference's sums, bit for bit, with
 // plain stores.  Above kLocalOwnerMax particles the terms are added
 // atomically (order not fixed where boxes overlap).
-template <class T, class F, int D>
+template <class T, int D>
 __device__ __forceinline__ void stamp_velocity(const T* P, T ts, T* vp) {
   for (int d = 0; d < D; ++d) vp[d] = (P[PF_POS + d] - P[PF_POS_OLD + d]) / ts;
 }
@@ -216,7 +201,7 @@ __global__ __launch_bounds__(256) void k_hlbm_stamp(ParticleArgs a) {
   const Box<T, D> b = aabb_of<T, D>(P, a);
   const T ts = T(1.0) / T(a.sub_steps == 0 ? uint64_t(1) : a.sub_steps);
   T vp[3] = {T(0), T(0), T(0)};
-  stamp_velocity<T, F, D>(P, ts, vp);
+  stamp_velocity<T, D>(P, ts, vp);
   const F* por = static_cast<const F*>(a.por);
   F* pn = static_cast<F*>(a.pn);
   F* pd = static_cast<F*>(a.pd);
@@ -294,7 +279,7 @@ __global__ __launch_bounds__(256) void k_hlbm_stamp(ParticleArgs a) {
           mine_added = true;
         }
         T vq[3] = {T(0), T(0), T(0)};
-        stamp_velocity<T, F, D>(parts + size_t(lq[e]) * kPartFields, ts, vq);
+        stamp_velocity<T, D>(parts + size_t(lq[e]) * kPartFields, ts, vq);
         add(vq);
       }
     } else {
@@ -311,7 +296,7 @@ __global__ __launch_bounds__(256) void k_hlbm_stamp(ParticleArgs a) {
           break;
         }
         T vq[3] = {T(0), T(0), T(0)};
-        stamp_velocity<T, F, D>(Pj, ts, vq);
+        stamp_velocity<T, D>(Pj, ts, vq);
         add(vq);
       }
     }
