@@ -6,6 +6,13 @@ x, tau = 0.6, from rest (f = w_i).  One step = one fused AA stream+collide
 launch over all 8,388,608 cells.  Inputs are resident in HBM before the
 timed region; the timed region is bracketed by stream synchronisation, and
 the device time comes from HIP events recorded on the lattice's own stream.
+Before the warm-up steps the measured HBM ceilings of the same GPU are taken
+(lbm_hbm_ceiling, ~0.1 s of plain copies) and reported beside the 8 TB/s
+peak, and lbm_autotune picks the step's launch shape on this lattice (its
+trial steps are undone).  Both run the GPU past the power-management
+transient of the first ~30 ms of load after idle, when the step's launches
+run up to 25% slower than sustained; --cold skips both and times the
+default shape from idle.
 
 Algorithmic bytes per cell and step: 2 * 19 * 4 = 152 (one read and one
 write of every population).  Prints one JSON line.
@@ -40,15 +47,25 @@ def build_c2(lattice, storage, ext, g=1e-6, tau=0.6):
     return lat
 
 
-def cpu_baseline(ext, steps=2, threads=None):
+def cpu_threads():
+    """Host threads for the CPU baseline: the process's CPU share (OMP_NUM_THREADS
+    where the environment sets it, else the cores this process may run on)."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0))
+
+
+def cpu_baseline(ext, seconds=10.0, threads=None):
     """The oracle (the reference's arithmetic restated in C++, OpenMP over
-    cells) on the host cores, on a bounded sample of the same workload."""
-    import numpy as np
+    cells) on the host cores, on a bounded sample of the same workload: as
+    many steps as fit in about `seconds` (at least 4), after one warm-up
+    step; then one step on one thread."""
     from lbm_mirror_amd import abi
     from oracle import oracle as orc
     from helpers import duct_flags_3d, c2_ops
     if threads is None:
-        threads = min(16, os.cpu_count() or 1)
+        threads = cpu_threads()
     s = orc.Sim(abi.D3Q19, abi.F32, ext)
     s.set_flags(duct_flags_3d(*ext))
     s.set_periodic(1)
@@ -56,9 +73,12 @@ def cpu_baseline(ext, steps=2, threads=None):
         s.set_operator(fl, op)
     s.init_equilibrium(1.0, (0.0, 0.0, 0.0))
     s.step(1, threads=threads)  # warm-up (first-touch)
-    t0 = time.perf_counter()
-    s.step(steps, threads=threads)
-    dt = time.perf_counter() - t0
+    steps, dt = 0, 0.0
+    while steps < 4 or dt < seconds:
+        t0 = time.perf_counter()
+        s.step(2, threads=threads)
+        dt += time.perf_counter() - t0
+        steps += 2
     t1 = time.perf_counter()
     s.step(1, threads=1)
     dt1 = time.perf_counter() - t1
@@ -71,11 +91,26 @@ def cpu_baseline(ext, steps=2, threads=None):
         pass
     return {"value": round(cells * steps / dt / 1e6, 2), "unit": "MLUPS", "cores": threads,
             "kind": "port",
-            "one_thread": round(cells / dt1 / 1e6, 2), "nproc": os.cpu_count(), "cpu_model": model,
-            "sample": f"D3Q19 FP32 {ext[0]}x{ext[1]}x{ext[2]}, {steps} steps after 1 warm-up, "
+            "one_thread": round(cells / dt1 / 1e6, 2), "nproc": os.cpu_count(),
+            "affinity": len(os.sched_getaffinity(0)), "cpu_model": model,
+            "sample": f"D3Q19 FP32 {ext[0]}x{ext[1]}x{ext[2]}, {steps} steps ({dt:.1f} s) after 1 warm-up, "
                       f"oracle (reference arithmetic, OpenMP) on {threads} threads; one_thread: 1 step "
                       f"on 1 thread"}
 
+
+def hbm_ceilings(L, device=0):
+    """Measured HBM ceilings on this GPU (lbm_hbm_ceiling), about 0.1 s of
+    passes: a float4 copy between two 640 MiB buffers and one in-place
+    nontemporal pass over 19 planes of 640 MiB (the even AA phase's access
+    pattern without arithmetic), each as (bytes read + written) / time."""
+    import ctypes as C
+    from lbm_mirror_amd import abi, check
+    out = {}
+    for key, kind, reps in (("copy_gbs", abi.CEILING_COPY, 200), ("planes_gbs", abi.CEILING_PLANES, 250)):
+        g = C.c_double()
+        check(L.lbm_hbm_ceiling(device, kind, 640 << 20, reps, C.byref(g)))
+        out[key] = round(g.value, 1)
+    return out
 
 
 def bench_c1(args):
@@ -537,7 +572,13 @@ def main():
     ap.add_argument("--lattice", default="D3Q19", choices=["D3Q19", "D3Q27"])
     ap.add_argument("--storage", default="F32", choices=["F32", "F64", "F16S", "BF16S"])
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0,
+                    help="CPU baseline: time budget of the timed oracle steps")
+    ap.add_argument("--cold", action="store_true",
+                    help="skip the measured ceilings and the launch-shape tuning before the warm-up "
+                         "(times the power-up transient with the default shape)")
+    ap.add_argument("--tune-rounds", type=int, default=12,
+                    help="passes of lbm_autotune over the launch shapes before the warm-up")
     ap.add_argument("--config", default="c2", choices=["c2", "c2b", "c1", "c3", "c4", "c5", "writers", "sphere3d"],
                     help="c2: the headline D3Q19 channel (default); c2b: D3Q27 256^3; "
                          "c1: D2Q9 FP64 channel; c3: particle-laden; c4: ragged batch; "
@@ -594,6 +635,18 @@ def main():
     print(f"[bench] building {args.lattice} {args.storage} {ext}", file=sys.stderr, flush=True)
     lat = build_c2(lat_id, st_id, ext)
     cells = ext[0] * ext[1] * ext[2]
+    # The measured copy ceilings go first: ~0.1 s of HBM load, after which the
+    # GPU runs at its sustained rate.  From idle, the first ~30 ms of load run
+    # up to 25% slower while the power management settles (per-launch trace
+    # in the headline profile); --cold times the steps without them.
+    ceil, tune = None, None
+    if not args.cold:
+        ceil = hbm_ceilings(L)
+        print(f"[bench] measured ceilings {ceil}", file=sys.stderr, flush=True)
+        # the launch shape (block width, addressing form per AA phase) timed on
+        # this lattice, state restored (lbm_autotune)
+        tune = lat.autotune(rounds=args.tune_rounds)
+        print(f"[bench] launch shape {tune}", file=sys.stderr, flush=True)
     if args.warmup:
         lat.step(args.warmup, t0=0)
     lat.sync()
@@ -633,16 +686,22 @@ def main():
         "config": {"workload": f"{args.lattice} BGK+Guo periodic-x channel, BB walls y/z, "
                                f"{ext[0]}x{ext[1]}x{ext[2]}, tau 0.6, g 1e-6",
                    "lattice": list(ext), "bytes_per_cell_step": bytes_per_cell,
+                   "launch_shape": tune or lat.get_step_tuning(),
                    "wall_s": round(wall, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic},
+                     "traffic": traffic,
+                     "traffic_source": "HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, "
+                                       "copy-calibrated, mean of the two AA phases "
+                                       f"(profiles/traffic.json, key {args.lattice}_{args.storage}); "
+                                       "not measured in this run" if traffic else None,
+                     "measured_ceiling": ceil and dict(ceil, frac_of_planes=round(achieved / ceil["planes_gbs"], 4))},
     }
     lat.close()
     if not args.no_cpu:
         print("[bench] CPU baseline (oracle)", file=sys.stderr, flush=True)
         try:
-            out["cpu_baseline"] = cpu_baseline(ext, steps=args.cpu_steps)
+            out["cpu_baseline"] = cpu_baseline(ext, seconds=args.cpu_seconds)
         except Exception as e:  # the baseline is reported, never the product
             out["cpu_baseline"] = {"error": str(e)}
     print(json.dumps(out), flush=True)

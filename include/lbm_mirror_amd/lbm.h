@@ -234,6 +234,42 @@ lbm_status lbm_sync(lbm_ctx* ctx);
  * context's stream around its launches. */
 lbm_status lbm_last_step_ms(lbm_ctx* ctx, double* ms);
 
+/* ---- launch shape of the fused step (no reference counterpart) -----------
+ * The fused stream+collide kernel runs one row segment of block_x cells per
+ * block, and addresses the population planes either with per-lane global
+ * addresses (form 1) or through one buffer resource with the plane as a
+ * scalar offset (form 2, where the array's bytes fit its 32-bit range);
+ * form 0 takes the lattice's default.  Results are bitwise the same in every
+ * shape and form.  lbm_autotune times every width (64, 128, 256, no wider
+ * than the row) in both forms per AA phase on the context's own lattice over
+ * `rounds` passes (a pass runs four even/odd step pairs back to back per
+ * candidate, each launch between HIP events on the context's stream; the
+ * mean per phase of a pass, the least over passes, is kept), restores
+ * the populations and the cell force and porosity fields it advanced, and
+ * keeps the fastest shape; `chosen->us` is the chosen shape's time per phase.
+ * Call it from an even step count, after flags and operators are set. */
+typedef struct lbm_step_tuning {
+  int32_t block_x;  /* threads per block along x: 64, 128 or 256     */
+  int32_t form[2];  /* per AA phase (even, odd): 0 default, 1 global, 2 buffer */
+  double us[2];     /* lbm_autotune: measured time per launch of each phase */
+} lbm_step_tuning;
+lbm_status lbm_autotune(lbm_ctx* ctx, uint32_t rounds, lbm_step_tuning* chosen);
+lbm_status lbm_set_step_tuning(lbm_ctx* ctx, const lbm_step_tuning* tuning);
+lbm_status lbm_get_step_tuning(lbm_ctx* ctx, lbm_step_tuning* tuning);
+
+/* ---- measured HBM ceiling (no reference counterpart) ----------------------
+ * The bandwidth plain data movement reaches on `device`, as a measured
+ * ceiling beside the published peak for the population kernels' roofline:
+ *   LBM_CEILING_COPY    b = a between two buffers of `bytes`, 16 B per lane;
+ *   LBM_CEILING_PLANES  one in-place nontemporal pass over 19 planes of
+ *                       `bytes` in total, 4 B of every plane per lane (the
+ *                       D3Q19 even AA phase without its arithmetic).
+ * One untimed pass, then `reps` passes timed with HIP events on a stream of
+ * its own; *gbps = (bytes read + bytes written) / mean pass time.  Allocates
+ * and frees its buffers; bytes in [1 MiB, 64 GiB]. */
+typedef enum lbm_ceiling_kind { LBM_CEILING_COPY = 0, LBM_CEILING_PLANES = 1 } lbm_ceiling_kind;
+lbm_status lbm_hbm_ceiling(int32_t device, int32_t kind, uint64_t bytes, uint32_t reps, double* gbps);
+
 /* ---- result output: write_vtk.hpp, write_csv.hpp, abstract/forst.hpp ------
  * `members` lists the fields of the struct being written, comma-separated, in
  * file order; each is one of the reference drivers' member keys

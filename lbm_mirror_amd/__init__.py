@@ -44,7 +44,8 @@ EXPORTS = [
     "lbm_net_get_output_scale", "lbm_net_set_output_scale",
     "lbm_get_flags", "lbm_ctx_info", "lbm_write_vtk", "lbm_write_csv", "lbm_write_forst",
     "lbm_read_forst", "lbm_write_vtk_fields", "lbm_write_csv_fields", "lbm_write_forst_fields",
-    "lbm_read_forst_fields", "lbm_set_step_graph",
+    "lbm_read_forst_fields", "lbm_set_step_graph", "lbm_hbm_ceiling",
+    "lbm_autotune", "lbm_set_step_tuning", "lbm_get_step_tuning",
 ]
 
 
@@ -121,6 +122,10 @@ def load_library(path=None):
     L.lbm_net_set_output_scale.argtypes = [vp, fp, fp]
     L.lbm_get_flags.argtypes = [vp, u8p]
     L.lbm_set_step_graph.argtypes = [vp, C.c_uint32]
+    L.lbm_hbm_ceiling.argtypes = [C.c_int32, C.c_int32, C.c_uint64, C.c_uint32, dp]
+    L.lbm_autotune.argtypes = [vp, C.c_uint32, C.POINTER(abi.StepTuning)]
+    L.lbm_set_step_tuning.argtypes = [vp, C.POINTER(abi.StepTuning)]
+    L.lbm_get_step_tuning.argtypes = [vp, C.POINTER(abi.StepTuning)]
     L.lbm_ctx_info.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), u64p]
     cs = C.c_char_p
     L.lbm_write_vtk.argtypes = [vp, cs, cs, C.c_uint64, cs]

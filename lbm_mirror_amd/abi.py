@@ -45,6 +45,13 @@ OPF_ZOUHE_NORMAL_ONLY = 0x10
 
 POROSITY_SCATTER, POROSITY_PRODUCT = 0, 1
 
+class StepTuning(C.Structure):
+    _fields_ = [("block_x", C.c_int32), ("form", C.c_int32 * 2), ("us", C.c_double * 2)]
+
+
+# lbm_ceiling_kind
+CEILING_COPY, CEILING_PLANES = 0, 1
+
 
 class OpDesc(C.Structure):
     _fields_ = [

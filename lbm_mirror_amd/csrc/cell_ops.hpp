@@ -30,18 +30,24 @@ struct OpDev {
 };
 
 // compute_rho_u (macroscopic.hpp:10-35)
+// The sums start from their first term instead of 0 + term (equal up to
+// the sign of an exact zero).
 template <class L, class T>
 __device__ __forceinline__ void rho_u(const T (&f)[L::Q], T& rho, T (&u)[3]) {
-  rho = T(0);
+  rho = f[0];
   u[0] = u[1] = u[2] = T(0);
+  bool seen[3] = {false, false, false};
 #pragma unroll
   for (int j = 0; j < L::Q; ++j) {
-    rho = rho + f[j];
+    if (j > 0) rho = rho + f[j];
 #pragma unroll
     for (int d = 0; d < L::D; ++d) {
       const int c = cdir<L>(j, d);
-      if (c == 1) u[d] = u[d] + f[j];
-      if (c == -1) u[d] = u[d] - f[j];
+      if (c == 0) continue;
+      if (!seen[d]) u[d] = c == 1 ? f[j] : -f[j];
+      else if (c == 1) u[d] = u[d] + f[j];
+      else u[d] = u[d] - f[j];
+      seen[d] = true;
     }
   }
 #pragma unroll
@@ -142,16 +148,25 @@ __device__ __forceinline__ void bgk(T (&f)[L::Q], T rho, const T (&u)[3], T om) 
 
 // BGK with Guo source (collision.hpp:280-315), F = cell force + global force.
 // Components outside `fmask` are exactly zero, and (t1+t2)·0 adds nothing to
-// the projection, so they are skipped.
+// the projection, so they are skipped.  Per direction the projection term of
+// component d is ((c_d − u_d)·3 + c_d·s2)·F_d with s2 = (c·u·3)·3; c_d is
+// +1, −1 or 0, so (c_d − u_d)·3 takes one of three values per cell (formed
+// once), c_d·s2 is s2, −s2 or a zero, and the pair's opposite uses the
+// other sign.  Each term is bitwise the reference's, up to the sign of an
+// exact zero (x + (±0) = x).
 template <class L, class T, bool WB>
 __device__ __forceinline__ void bgk_guo(T (&f)[L::Q], T rho, const T (&u)[3], T om,
                                         const T (&F)[3], uint32_t fmask) {
   const T vv = vel_sq<L, T>(u);
   const T relax = T(1) - T(0.5f) * om;
   const T ics2 = T(3.0);
-  T u3[3];
+  T up[3], um[3], u0[3];  // (1 − u_d)·3, (−1 − u_d)·3, (0 − u_d)·3
 #pragma unroll
-  for (int d = 0; d < L::D; ++d) u3[d] = u[d];
+  for (int d = 0; d < L::D; ++d) {
+    up[d] = (T(1) - u[d]) * ics2;
+    um[d] = (T(-1) - u[d]) * ics2;
+    u0[d] = (T(0) - u[d]) * ics2;
+  }
 #pragma unroll
   for (int i = 0; i < L::Q; ++i) {
     if (!pair_head<L>(i)) continue;
@@ -160,15 +175,26 @@ __device__ __forceinline__ void bgk_guo(T (&f)[L::Q], T rho, const T (&u)[3], T 
     const T a = cu * ics2;
     T ei, eo;
     eq_pair<L, T, WB>(i, rho, a, vv, ei, eo);
-    const T s2 = a * ics2;  // (c·u·3)·3; the opposite's is −s2, and c·s2 is shared
+    const T s2 = a * ics2;  // (c·u·3)·3; the opposite's is −s2
     T pi = T(0), po = T(0);
+    bool first = true;
 #pragma unroll
     for (int d = 0; d < L::D; ++d) {
       if (!(fmask & (1u << d))) continue;
-      const T cd = T(cdir<L>(i, d));
-      const T t2 = cd * s2;
-      pi = pi + ((cd - u3[d]) * ics2 + t2) * F[d];
-      po = po + (((-cd) - u3[d]) * ics2 + t2) * F[d];
+      const int cd = cdir<L>(i, d);
+      T ti, to;
+      if (cd == 1) {
+        ti = (up[d] + s2) * F[d];
+        to = (um[d] + s2) * F[d];
+      } else if (cd == -1) {
+        ti = (um[d] - s2) * F[d];
+        to = (up[d] - s2) * F[d];
+      } else {
+        ti = to = u0[d] * F[d];
+      }
+      pi = first ? ti : pi + ti;
+      po = first ? to : po + to;
+      first = false;
     }
     const T wi = T(L::w[i]);
     f[i] = (f[i] + om * (ei - f[i])) + (wi * pi) * relax;

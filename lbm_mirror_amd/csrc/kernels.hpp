@@ -34,6 +34,10 @@ struct StepArgs {
 struct LaunchShape {
   int bx = 256, by = 1;  // threads per block along x and y (scalar kernel)
   bool vec = true;       // vectorised kernel where the row length allows it
+  // population addressing per AA phase: 0 the lattice's default, 1 global
+  // addresses, 2 one buffer resource (used only where the array's bytes fit
+  // its 32-bit range)
+  int32_t form[2] = {0, 0};
 };
 
 hipError_t launch_aa_step(int lattice, int storage, bool odd, const StepArgs& a,

@@ -790,6 +790,162 @@ lbm_status lbm_last_step_ms(lbm_ctx* c, double* ms) {
   return LBM_OK;
 }
 
+// ---------------------------------------------------------------- step tuning
+lbm_status lbm_set_step_tuning(lbm_ctx* c, const lbm_step_tuning* t) {
+  if (!c || !t) return fail(LBM_ERR_INVALID_ARG, "null argument");
+  if (t->block_x != 64 && t->block_x != 128 && t->block_x != 256)
+    return fail(LBM_ERR_INVALID_ARG, "block_x must be 64, 128 or 256");
+  for (int p = 0; p < 2; ++p)
+    if (t->form[p] < 0 || t->form[p] > 2) return fail(LBM_ERR_INVALID_ARG, "form must be 0, 1 or 2");
+  c->shape.bx = t->block_x;
+  c->shape.by = 1;
+  c->shape.form[0] = t->form[0];
+  c->shape.form[1] = t->form[1];
+  return LBM_OK;
+}
+
+lbm_status lbm_autotune(lbm_ctx* c, uint32_t rounds, lbm_step_tuning* chosen) {
+  if (!c) return fail(LBM_ERR_INVALID_ARG, "null ctx");
+  if (rounds == 0 || rounds > 64) return fail(LBM_ERR_INVALID_ARG, "rounds must be in [1, 64]");
+  lbm_status s = check_ready(c);
+  if (s) return s;
+  if (c->phase != 0) return fail(LBM_ERR_INVALID_STATE, "autotune from the natural layout (even step count)");
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  // the trial steps advance the populations and may write the cell force and
+  // porosity fields (PSM variants): those are saved and restored; the macros
+  // are not written by the trials
+  struct Saved {
+    void* field;
+    void* copy;
+    size_t bytes;
+  };
+  std::vector<Saved> saved;
+  auto release = [&]() {
+    for (Saved& v : saved)
+      if (v.copy) (void)hipFree(v.copy);
+  };
+  const size_t fbytes = size_t(c->Q) * c->Np * storage_bytes(c->storage);
+  const std::pair<void*, size_t> fields[] = {{c->f, fbytes},
+                                             {c->force, size_t(c->D) * c->N * c->tsz()},
+                                             {c->por, size_t(c->N) * c->tsz()}};
+  for (const auto& fl : fields) {
+    if (!fl.first) continue;
+    Saved v{fl.first, nullptr, fl.second};
+    hipError_t e = hipMalloc(&v.copy, v.bytes);
+    if (e == hipSuccess) e = hipMemcpyAsync(v.copy, v.field, v.bytes, hipMemcpyDeviceToDevice, c->stream);
+    saved.push_back(v);
+    if (e != hipSuccess) {
+      release();
+      return fail(e == hipErrorOutOfMemory ? LBM_ERR_OUT_OF_MEMORY : LBM_ERR_CRITICAL, "autotune save: %s",
+                  hipGetErrorString(e));
+    }
+  }
+  // each candidate runs kPairs even/odd pairs back to back (the sustained
+  // rate, not that of isolated launches), every launch between two events
+  constexpr int kPairs = 4;
+  hipEvent_t ev[2 * kPairs + 1] = {};
+  auto cleanup = [&]() {
+    (void)hipStreamSynchronize(c->stream);
+    for (hipEvent_t e : ev)
+      if (e) (void)hipEventDestroy(e);
+    release();
+  };
+#define TUNE_TRY(expr)                                                                   \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess) {                                                              \
+      cleanup();                                                                         \
+      return fail(LBM_ERR_CRITICAL, "autotune: %s: %s", #expr, hipGetErrorString(e_));   \
+    }                                                                                    \
+  } while (0)
+  for (hipEvent_t& e : ev) TUNE_TRY(hipEventCreate(&e));
+  // candidates: one-row blocks of 64, 128 or 256 threads (no wider than the
+  // row needs), each phase in both addressing forms
+  std::vector<int> widths;
+  for (int bx = 64; bx <= 256; bx *= 2) {
+    widths.push_back(bx);
+    if (uint64_t(bx) >= c->n[0]) break;
+  }
+  const int nw = int(widths.size());
+  std::vector<double> best_us(size_t(nw) * 2 * 2, 1e30);  // [width][form-1][phase]
+  const LaunchShape keep = c->shape;
+  for (uint32_t r = 0; r < rounds; ++r)
+    for (int wi = 0; wi < nw; ++wi)
+      for (int form = 1; form <= 2; ++form) {
+        LaunchShape sh = keep;
+        sh.bx = widths[wi];
+        sh.by = 1;
+        sh.form[0] = sh.form[1] = form;
+        const StepArgs a = step_args(c, 0, false);
+        TUNE_TRY(hipEventRecord(ev[0], c->stream));
+        for (int k = 0; k < kPairs; ++k) {
+          TUNE_TRY(launch_aa_step(c->lattice, c->storage, false, a, sh, c->stream));
+          TUNE_TRY(hipEventRecord(ev[2 * k + 1], c->stream));
+          TUNE_TRY(launch_aa_step(c->lattice, c->storage, true, a, sh, c->stream));
+          TUNE_TRY(hipEventRecord(ev[2 * k + 2], c->stream));
+        }
+        TUNE_TRY(hipEventSynchronize(ev[2 * kPairs]));
+        double sum[2] = {0.0, 0.0};
+        for (int k = 0; k < 2 * kPairs; ++k) {
+          float m = 0.f;
+          TUNE_TRY(hipEventElapsedTime(&m, ev[k], ev[k + 1]));
+          sum[k & 1] += 1e3 * double(m);
+        }
+        double* b = &best_us[(size_t(wi) * 2 + (form - 1)) * 2];
+        b[0] = std::min(b[0], sum[0] / kPairs);
+        b[1] = std::min(b[1], sum[1] / kPairs);
+      }
+  // restore the state the trials advanced
+  for (const Saved& v : saved)
+    TUNE_TRY(hipMemcpyAsync(v.field, v.copy, v.bytes, hipMemcpyDeviceToDevice, c->stream));
+  TUNE_TRY(hipStreamSynchronize(c->stream));
+#undef TUNE_TRY
+  cleanup();
+  // per width the faster form of each phase; the width with the least total
+  int bw = 0, bf[2] = {1, 1};
+  double bsum = 1e30, bus[2] = {0.0, 0.0};
+  for (int wi = 0; wi < nw; ++wi) {
+    int f[2];
+    double u[2];
+    for (int p = 0; p < 2; ++p) {
+      const double g = best_us[(size_t(wi) * 2 + 0) * 2 + p], b = best_us[(size_t(wi) * 2 + 1) * 2 + p];
+      f[p] = b < g ? 2 : 1;
+      u[p] = std::min(g, b);
+    }
+    if (u[0] + u[1] < bsum) {
+      bsum = u[0] + u[1];
+      bw = wi;
+      bf[0] = f[0];
+      bf[1] = f[1];
+      bus[0] = u[0];
+      bus[1] = u[1];
+    }
+  }
+  c->shape = keep;
+  c->shape.bx = widths[bw];
+  c->shape.by = 1;
+  c->shape.form[0] = bf[0];
+  c->shape.form[1] = bf[1];
+  if (chosen) {
+    chosen->block_x = widths[bw];
+    chosen->form[0] = bf[0];
+    chosen->form[1] = bf[1];
+    chosen->us[0] = bus[0];
+    chosen->us[1] = bus[1];
+  }
+  return LBM_OK;
+}
+
+lbm_status lbm_get_step_tuning(lbm_ctx* c, lbm_step_tuning* t) {
+  if (!c || !t) return fail(LBM_ERR_INVALID_ARG, "null argument");
+  t->block_x = c->shape.bx;
+  t->form[0] = c->shape.form[0];
+  t->form[1] = c->shape.form[1];
+  t->us[0] = t->us[1] = 0.0;
+  return LBM_OK;
+}
+
 // ---------------------------------------------------------------- particles
 lbm_status lbm_spheroid_mass(int32_t dims, int32_t storage, double r, double rho, double* mass,
                              double* inertia) {

@@ -407,17 +407,22 @@ template <>
 struct StepWaves<D2Q9, float> {
   static constexpr int value = LBMX_WAVES_Q9;
 };
-template <class L, class S, class T, bool WB, bool ODD, bool BUF>
+// ROW1: every block is one row segment (blockDim.y == 1), so y and z, the
+// offsets of the neighbour rows and their validity are wavefront-uniform
+// (scalar registers) and each population access is a uniform row base plus
+// a 32-bit per-lane x offset; only x and its two neighbours live per lane.
+template <class L, class S, class T, bool WB, bool ODD, bool BUF, bool ROW1>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(StepWaves<L, T>::value))) void k_aa_step(
     StepArgs a) {
   constexpr int Q = L::Q;
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  const int y = blockIdx.y * blockDim.y + threadIdx.y;
+  const int y = ROW1 ? int(blockIdx.y) : int(blockIdx.y * blockDim.y + threadIdx.y);
   const int z = blockIdx.z;
   if (x >= a.nx || y >= a.ny) return;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const uint32_t sxy = uint32_t(nx) * uint32_t(ny);
-  const uint32_t cell = uint32_t(x) + uint32_t(nx) * (uint32_t(y) + uint32_t(ny) * uint32_t(z));
+  const uint32_t row = uint32_t(nx) * (uint32_t(y) + uint32_t(ny) * uint32_t(z));
+  const uint32_t cell = row + uint32_t(x);
   S* __restrict__ fb = static_cast<S*>(a.f);
   const uint32_t Np = a.Np;
 
@@ -433,60 +438,65 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(StepWaves<L
   const bool VY[3] = {y > 0 || py, true, y < ny - 1 || py};
   const bool VZ[3] = {L::D < 3 || z > 0 || pz, true, L::D < 3 || z < nz - 1 || pz};
 
-  // plane p, element e: load and store in this phase's cache policy
+  // plane p, row offset r, x offset e: load and store in this phase's cache
+  // policy (r joins the plane into the scalar part of the address when ROW1)
   const __amdgpu_buffer_rsrc_t rs =
       __builtin_amdgcn_make_buffer_rsrc(a.f, 0, int(uint32_t(Q) * Np * uint32_t(sizeof(S))), 0x00020000);
-  auto ld = [&](int p, uint32_t e, T w) -> T {
-    if constexpr (BUF)
+  // (a buffer access's scalar offset must be uniform: without ROW1 the row
+  // offset joins the per-lane part)
+  auto ld = [&](int p, uint32_t r, uint32_t e, T w) -> T {
+    if constexpr (BUF && ROW1)
       return BufIO<S, T>::template ld<ODD ? aux_odd_ld<S, T>() : aux_even_ld<S, T>()>(
-          rs, e, uint32_t(p) * Np * uint32_t(sizeof(S)), w);
+          rs, e, (uint32_t(p) * Np + r) * uint32_t(sizeof(S)), w);
+    else if constexpr (BUF)
+      return BufIO<S, T>::template ld<ODD ? aux_odd_ld<S, T>() : aux_even_ld<S, T>()>(
+          rs, r + e, uint32_t(p) * Np * uint32_t(sizeof(S)), w);
     else if constexpr (ODD)
-      return Store<S, T>::ld(fb + size_t(p) * Np + e, w);
+      return Store<S, T>::ld(fb + (size_t(p) * Np + r) + e, w);
     else
-      return ld_even<S, T>(fb + size_t(p) * Np + e, w);
+      return ld_even<S, T>(fb + (size_t(p) * Np + r) + e, w);
   };
-  auto st = [&](int p, uint32_t e, T v, T w) {
-    if constexpr (BUF)
+  auto st = [&](int p, uint32_t r, uint32_t e, T v, T w) {
+    if constexpr (BUF && ROW1)
       BufIO<S, T>::template st<ODD ? aux_odd_st<S, T>() : aux_even_st<S, T>()>(
-          rs, e, uint32_t(p) * Np * uint32_t(sizeof(S)), v, w);
+          rs, e, (uint32_t(p) * Np + r) * uint32_t(sizeof(S)), v, w);
+    else if constexpr (BUF)
+      BufIO<S, T>::template st<ODD ? aux_odd_st<S, T>() : aux_even_st<S, T>()>(
+          rs, r + e, uint32_t(p) * Np * uint32_t(sizeof(S)), v, w);
     else if constexpr (ODD)
-      Store<S, T>::st(fb + size_t(p) * Np + e, v, w);
+      Store<S, T>::st(fb + (size_t(p) * Np + r) + e, v, w);
     else
-      st_even<S, T>(fb + size_t(p) * Np + e, v, w);
+      st_even<S, T>(fb + (size_t(p) * Np + r) + e, v, w);
   };
 
   T f[Q];
   if (!ODD) {
 #pragma unroll
-    for (int i = 0; i < Q; ++i) f[i] = ld(i, cell, T(L::w[i]));
+    for (int i = 0; i < Q; ++i) f[i] = ld(i, row, uint32_t(x), T(L::w[i]));
   } else {
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
       const int ix = 1 - L::cx[i], iy = 1 - L::cy[i], iz = 1 - L::cz[i];  // x − c_i
-      if (VX[ix] && VY[iy] && VZ[iz]) {
-        const uint32_t src = X[ix] + Y[iy] + Z[iz];
-        f[i] = ld(L::opp[i], src, T(L::w[i]));
-      } else {
+      if (VX[ix] && VY[iy] && VZ[iz])
+        f[i] = ld(L::opp[i], Y[iy] + Z[iz], X[ix], T(L::w[i]));
+      else
         f[i] = T(0);
-      }
     }
   }
 
-  collide_cell<L, T, WB>(f, a.flags[cell], a, cell, x, y, z);
+  collide_cell<L, T, WB>(f, ((const __attribute__((address_space(1))) uint8_t*)a.flags + row)[x], a, cell, x,
+                         y, z);
 
   if (!ODD) {
 #pragma unroll
-    for (int i = 0; i < Q; ++i) st(L::opp[i], cell, f[i], T(L::w[i]));
+    for (int i = 0; i < Q; ++i) st(L::opp[i], row, uint32_t(x), f[i], T(L::w[i]));
   } else {
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
       const int ix = 1 + L::cx[i], iy = 1 + L::cy[i], iz = 1 + L::cz[i];  // x + c_i
-      if (VX[ix] && VY[iy] && VZ[iz]) {
-        const uint32_t dst = X[ix] + Y[iy] + Z[iz];
-        st(i, dst, f[i], T(L::w[i]));
-      }
+      if (VX[ix] && VY[iy] && VZ[iz]) st(i, Y[iy] + Z[iz], X[ix], f[i], T(L::w[i]));
       const int jx = 1 - L::cx[i], jy = 1 - L::cy[i], jz = 1 - L::cz[i];
-      if (!(VX[jx] && VY[jy] && VZ[jz])) st(i, cell, T(0), T(L::w[i]));
+      if (!(VX[jx] && VY[jy] && VZ[jz])) st(i, row, uint32_t(x), T(0), T(L::w[i]));
     }
   }
 }
@@ -961,13 +971,22 @@ static hipError_t launch_t(bool odd, const StepArgs& a, const LaunchShape& sh, h
   dim3 block(sh.bx, sh.by, 1);
   dim3 grid((a.nx + sh.bx - 1) / sh.bx, (a.ny + sh.by - 1) / sh.by, a.nz);
   // the buffer-resource form where the population array's bytes fit the 32-bit range
-  const bool buf = (buf_phases<L, S>() & (odd ? 2 : 1)) != 0 && uint64_t(L::Q) * a.Np * sizeof(S) <= 0xffffffffull;
-  if (odd) {
-    if (buf) hipLaunchKernelGGL((k_aa_step<L, S, T, WB, true, true>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((k_aa_step<L, S, T, WB, true, false>), grid, block, 0, s, a);
+  const int form = sh.form[odd ? 1 : 0];
+  const bool want_buf = form == 0 ? (buf_phases<L, S>() & (odd ? 2 : 1)) != 0 : form == 2;
+  const bool buf = want_buf && uint64_t(L::Q) * a.Np * sizeof(S) <= 0xffffffffull;
+  // one-row blocks take the row-uniform addressing in the global form
+  // (D3Q19 FP32 1%, D2Q9 FP64 10% faster); the buffer form keeps per-lane
+  // rows (row-uniform scalar offsets are 4% slower in D3Q27 phase 1, 3% in
+  // D2Q9 FP32)
+  if (buf) {
+    if (odd) hipLaunchKernelGGL((k_aa_step<L, S, T, WB, true, true, false>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((k_aa_step<L, S, T, WB, false, true, false>), grid, block, 0, s, a);
+  } else if (sh.by == 1) {
+    if (odd) hipLaunchKernelGGL((k_aa_step<L, S, T, WB, true, false, true>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((k_aa_step<L, S, T, WB, false, false, true>), grid, block, 0, s, a);
   } else {
-    if (buf) hipLaunchKernelGGL((k_aa_step<L, S, T, WB, false, true>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((k_aa_step<L, S, T, WB, false, false>), grid, block, 0, s, a);
+    if (odd) hipLaunchKernelGGL((k_aa_step<L, S, T, WB, true, false, false>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((k_aa_step<L, S, T, WB, false, false, false>), grid, block, 0, s, a);
   }
   return hipGetLastError();
 }

@@ -146,6 +146,24 @@ class Lattice:
         check(self.L.lbm_last_step_ms(self._h, C.byref(m)))
         return m.value
 
+    def autotune(self, rounds=8):
+        """Time the fused step's launch shapes on this lattice and keep the
+        fastest (lbm_autotune); returns the chosen shape and its times."""
+        t = abi.StepTuning()
+        check(self.L.lbm_autotune(self._h, rounds, C.byref(t)))
+        return {"block_x": t.block_x, "form": [t.form[0], t.form[1]], "us": [t.us[0], t.us[1]]}
+
+    def set_step_tuning(self, block_x, form=(0, 0)):
+        t = abi.StepTuning()
+        t.block_x = block_x
+        t.form[0], t.form[1] = form
+        check(self.L.lbm_set_step_tuning(self._h, C.byref(t)))
+
+    def get_step_tuning(self):
+        t = abi.StepTuning()
+        check(self.L.lbm_get_step_tuning(self._h, C.byref(t)))
+        return {"block_x": t.block_x, "form": [t.form[0], t.form[1]]}
+
     # ---- particles --------------------------------------------------------
     def particles_set(self, parts, params):
         arr = (abi.Particle * max(1, len(parts)))(*parts)

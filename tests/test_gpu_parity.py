@@ -121,7 +121,9 @@ def test_d3q27_zouhe_pressure_channel(hip, orc, st, zflags):
                                      # wavefront), and 65 threads per row (a row over
                                      # two wavefronts, one active lane in the second)
                                      (abi.D2Q9, (20, 7)), (abi.D2Q9, (260, 5)),
-                                     (abi.D3Q19, (12, 5, 4)), (abi.D3Q27, (260, 3, 3))])
+                                     (abi.D3Q19, (12, 5, 4)), (abi.D3Q27, (260, 3, 3)),
+                                     # one-row blocks (row-uniform addressing)
+                                     (abi.D3Q19, (260, 3, 3))])
 @pytest.mark.parametrize("periodic", [0, 3])
 def test_random_flags_every_operator(hip, orc, lat, ext, periodic):
     """Random flag mixture over every operator the lattice supports, random
