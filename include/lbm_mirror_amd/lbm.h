@@ -45,7 +45,8 @@ typedef enum lbm_status {
   LBM_ERR_INVALID_STATE = 4,  /* err::invalid_state — call order / NaN      */
   LBM_ERR_NOT_SUPPORTED = 5,  /* err::not_supported                         */
   LBM_ERR_INVALID_ARG = 6,    /* err::invalid_argument                      */
-  LBM_ERR_OVERFLOW = 7        /* cell-list bin overflow (cell_list.hpp:99)  */
+  LBM_ERR_OVERFLOW = 7        /* err::buffer_exhausted: a cell-list bin was
+                                 full (cell_list.hpp:96-110)                */
 } lbm_status;
 
 /* ---- lattices: descriptor/{d2q9,d3q19,d3q27}.hpp ------------------------ */
@@ -203,6 +204,19 @@ lbm_status lbm_get_hlbm_sums(lbm_ctx* ctx, double* n, double* d);
  * (t odd) — whatever the physical AA phase is. */
 lbm_status lbm_set_populations(lbm_ctx* ctx, const double* f);
 lbm_status lbm_get_populations(lbm_ctx* ctx, double* f);
+/* The same populations in a chosen host layout:
+ *  LBM_LAYOUT_SOA            f[i*N + cell] (as above), Q*N doubles;
+ *  LBM_LAYOUT_REF_AOS_GHOST  the reference's dfs field, a
+ *    Chunk<FixedArray<T,Q>, 1, nx, ny[, nz]> (chunk.hpp:11-24, flatten.hpp:
+ *    11-39): cells of the ghost-widened extent x-fastest, cell (x, y, z) of
+ *    the lattice at ((x+1) + (nx+2)*((y+1) + (ny+2)*(z+1))), the Q values of
+ *    a cell consecutive; Q*(nx+2)*(ny+2)[*(nz+2)] doubles.  Get writes the
+ *    ghost ring as zeros (the reference's ghost values); set ignores it.
+ * lbm_population_count gives the array length a layout takes. */
+typedef enum lbm_layout { LBM_LAYOUT_SOA = 0, LBM_LAYOUT_REF_AOS_GHOST = 1 } lbm_layout;
+lbm_status lbm_population_count(lbm_ctx* ctx, int32_t layout, uint64_t* count);
+lbm_status lbm_set_populations_layout(lbm_ctx* ctx, const double* f, int32_t layout);
+lbm_status lbm_get_populations_layout(lbm_ctx* ctx, double* f, int32_t layout);
 /* Fill every cell with eq(ρ, u) in storage precision (init of the drivers,
  * poiseulle_2d_gpu.cpp:104-134) and reset macros to (ρ, u). */
 lbm_status lbm_init_equilibrium(lbm_ctx* ctx, double rho, const double u[3]);
@@ -229,7 +243,33 @@ lbm_status lbm_step(lbm_ctx* ctx, uint64_t t0, uint64_t nsteps);
  * runs of steps are replayed from one captured graph (recaptured whenever
  * an argument it captured changes); results are bit-identical either way. */
 lbm_status lbm_set_step_graph(lbm_ctx* ctx, uint32_t steps);
+/* Waits for the context's queued work.  Kernel-side faults are kept in a
+ * device status word (the reference's kernels have no error channel and
+ * return early, e.g. particle/aabb.hpp:25-40) and reported here, then
+ * cleared:
+ *   LBM_ERR_INVALID_STATE  a density or velocity the collision wrote was not
+ *                          finite (checked on every macro-writing step: the
+ *                          last step of each lbm_step call, or every step
+ *                          where the macros are kept each step);
+ *   LBM_ERR_OVERFLOW       a cell-list bin was full (product porosity mode;
+ *                          the count is in lbm_cell_list_overflow).
+ * lbm_step reports a fault of an earlier call's finished work the same way
+ * (without waiting for unfinished work), before it queues anything. */
 lbm_status lbm_sync(lbm_ctx* ctx);
+/* FP64 steady-state residual (no reference counterpart; the batch calls
+ * use the same definition): with tracking on, each lbm_step call also keeps
+ * the velocity of the step before its last, and lbm_residual gives
+ *   out[0] = r = |u_t - u_{t-1}|_2 / |u_t|_2  (0 when both are 0, NaN when
+ *            a sum is not finite),
+ *   out[1] = sum over cells and components of (u_t - u_{t-1})^2,
+ *   out[2] = sum of u_t^2,
+ * summed in FP64 on the GPU in a fixed order (blocks of 2048 cells, then the
+ * block sums), so equal inputs give equal bits.  u_t is the velocity field
+ * the last step wrote; u_{t-1} the one the step before wrote (for a
+ * one-step call, the previous call's last step).  lbm_set_residual
+ * allocates (enable = 1) or frees (0) the second velocity field. */
+lbm_status lbm_set_residual(lbm_ctx* ctx, int32_t enable);
+lbm_status lbm_residual(lbm_ctx* ctx, double out[3]);
 /* Device time of the last lbm_step call, from HIP events recorded on the
  * context's stream around its launches. */
 lbm_status lbm_last_step_ms(lbm_ctx* ctx, double* ms);

@@ -46,6 +46,8 @@ EXPORTS = [
     "lbm_read_forst", "lbm_write_vtk_fields", "lbm_write_csv_fields", "lbm_write_forst_fields",
     "lbm_read_forst_fields", "lbm_set_step_graph", "lbm_hbm_ceiling",
     "lbm_autotune", "lbm_set_step_tuning", "lbm_get_step_tuning",
+    "lbm_population_count", "lbm_set_populations_layout", "lbm_get_populations_layout",
+    "lbm_set_residual", "lbm_residual",
 ]
 
 
@@ -126,6 +128,11 @@ def load_library(path=None):
     L.lbm_autotune.argtypes = [vp, C.c_uint32, C.POINTER(abi.StepTuning)]
     L.lbm_set_step_tuning.argtypes = [vp, C.POINTER(abi.StepTuning)]
     L.lbm_get_step_tuning.argtypes = [vp, C.POINTER(abi.StepTuning)]
+    L.lbm_population_count.argtypes = [vp, C.c_int32, C.POINTER(C.c_uint64)]
+    L.lbm_set_populations_layout.argtypes = [vp, dp, C.c_int32]
+    L.lbm_get_populations_layout.argtypes = [vp, dp, C.c_int32]
+    L.lbm_set_residual.argtypes = [vp, C.c_int32]
+    L.lbm_residual.argtypes = [vp, dp]
     L.lbm_ctx_info.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), u64p]
     cs = C.c_char_p
     L.lbm_write_vtk.argtypes = [vp, cs, cs, C.c_uint64, cs]

@@ -49,6 +49,9 @@ class StepTuning(C.Structure):
     _fields_ = [("block_x", C.c_int32), ("form", C.c_int32 * 2), ("us", C.c_double * 2)]
 
 
+# lbm_layout
+LAYOUT_SOA, LAYOUT_REF_AOS_GHOST = 0, 1
+
 # lbm_ceiling_kind
 CEILING_COPY, CEILING_PLANES = 0, 1
 

@@ -47,8 +47,22 @@ struct Shift<__hip_bfloat16> {
   static constexpr bool on = true;
 };
 
+// Index of population i of cell (x, y, z) in a host-side population array:
+// SoA doubles [i*N + cell], or the reference's chunk layout (chunk.hpp:
+// 11-24, flatten.hpp:11-39): a ghost ring of one cell on every side, cells
+// x-fastest over (nx+2)(ny+2)[(nz+2)], the Q populations of a cell
+// consecutive (FixedArray<T,Q> per cell).
+template <int D, bool AOS>
+__device__ inline size_t pop_index(const Geometry& g, int x, int y, int z, uint32_t cell, int i, int Q) {
+  if (!AOS) return size_t(i) * g.N + cell;
+  const size_t gx = size_t(g.nx) + 2, gy = size_t(g.ny) + 2;
+  const size_t gz = D == 3 ? size_t(z) + 1 : 0;
+  return ((size_t(x) + 1) + gx * ((size_t(y) + 1) + gy * gz)) * size_t(Q) + size_t(i);
+}
+
 // logical f_i(x) of the current state, natural SoA doubles out[i*N + cell]
-template <class L, class S>
+// (or the chunk layout; its ghost ring is left as the caller cleared it)
+template <class L, class S, bool AOS>
 __global__ void k_unpack(const S* __restrict__ f, double* __restrict__ out, Geometry g, int phase) {
   const uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x;
   if (cell >= g.N) return;
@@ -74,18 +88,19 @@ __global__ void k_unpack(const S* __restrict__ f, double* __restrict__ out, Geom
         v = 0.0;
       }
     }
-    out[size_t(i) * g.N + cell] = v;
+    out[pop_index<L::D, AOS>(g, x, y, z, cell, i, L::Q)] = v;
   }
 }
 
-// natural layout from doubles (phase 0)
-template <class L, class S>
+// natural layout from doubles (phase 0); the chunk layout's ghost ring is not read
+template <class L, class S, bool AOS>
 __global__ void k_pack(S* __restrict__ f, const double* __restrict__ in, Geometry g) {
   const uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x;
   if (cell >= g.N) return;
+  const int x = cell % g.nx, y = (cell / g.nx) % g.ny, z = cell / (uint32_t(g.nx) * g.ny);
 #pragma unroll
   for (int i = 0; i < L::Q; ++i) {
-    double v = in[size_t(i) * g.N + cell];
+    double v = in[pop_index<L::D, AOS>(g, x, y, z, cell, i, L::Q)];
     if (Shift<S>::on) v = double(float(v) - float(L::w[i]));
     f[size_t(i) * g.Np + cell] = from_d<S>(v);
   }
@@ -102,47 +117,57 @@ __global__ void k_from_double(T* __restrict__ a, const double* __restrict__ b, s
   if (i < n) a[i] = T(b[i]);
 }
 
-template <class L>
+template <class L, bool AOS>
 static hipError_t unpack_l(int storage, const void* f, double* out, const Geometry& g, int phase,
                            hipStream_t s) {
   dim3 b(256), gr((g.N + 255) / 256);
   switch (storage) {
-    case LBM_F64: hipLaunchKernelGGL((k_unpack<L, double>), gr, b, 0, s, (const double*)f, out, g, phase); break;
-    case LBM_F32: hipLaunchKernelGGL((k_unpack<L, float>), gr, b, 0, s, (const float*)f, out, g, phase); break;
-    case LBM_F16S: hipLaunchKernelGGL((k_unpack<L, __half>), gr, b, 0, s, (const __half*)f, out, g, phase); break;
-    case LBM_BF16S: hipLaunchKernelGGL((k_unpack<L, __hip_bfloat16>), gr, b, 0, s, (const __hip_bfloat16*)f, out, g, phase); break;
+    case LBM_F64: hipLaunchKernelGGL((k_unpack<L, double, AOS>), gr, b, 0, s, (const double*)f, out, g, phase); break;
+    case LBM_F32: hipLaunchKernelGGL((k_unpack<L, float, AOS>), gr, b, 0, s, (const float*)f, out, g, phase); break;
+    case LBM_F16S: hipLaunchKernelGGL((k_unpack<L, __half, AOS>), gr, b, 0, s, (const __half*)f, out, g, phase); break;
+    case LBM_BF16S: hipLaunchKernelGGL((k_unpack<L, __hip_bfloat16, AOS>), gr, b, 0, s, (const __hip_bfloat16*)f, out, g, phase); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
-template <class L>
+template <class L, bool AOS>
 static hipError_t pack_l(int storage, void* f, const double* in, const Geometry& g, hipStream_t s) {
   dim3 b(256), gr((g.N + 255) / 256);
   switch (storage) {
-    case LBM_F64: hipLaunchKernelGGL((k_pack<L, double>), gr, b, 0, s, (double*)f, in, g); break;
-    case LBM_F32: hipLaunchKernelGGL((k_pack<L, float>), gr, b, 0, s, (float*)f, in, g); break;
-    case LBM_F16S: hipLaunchKernelGGL((k_pack<L, __half>), gr, b, 0, s, (__half*)f, in, g); break;
-    case LBM_BF16S: hipLaunchKernelGGL((k_pack<L, __hip_bfloat16>), gr, b, 0, s, (__hip_bfloat16*)f, in, g); break;
+    case LBM_F64: hipLaunchKernelGGL((k_pack<L, double, AOS>), gr, b, 0, s, (double*)f, in, g); break;
+    case LBM_F32: hipLaunchKernelGGL((k_pack<L, float, AOS>), gr, b, 0, s, (float*)f, in, g); break;
+    case LBM_F16S: hipLaunchKernelGGL((k_pack<L, __half, AOS>), gr, b, 0, s, (__half*)f, in, g); break;
+    case LBM_BF16S: hipLaunchKernelGGL((k_pack<L, __hip_bfloat16, AOS>), gr, b, 0, s, (__hip_bfloat16*)f, in, g); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
+size_t population_count(int lattice, const Geometry& g, int layout) {
+  const int Q = lattice == LBM_D2Q9 ? 9 : (lattice == LBM_D3Q19 ? 19 : 27);
+  if (layout != LBM_LAYOUT_REF_AOS_GHOST) return size_t(Q) * g.N;
+  size_t cells = (size_t(g.nx) + 2) * (size_t(g.ny) + 2);
+  if (lattice != LBM_D2Q9) cells *= size_t(g.nz) + 2;
+  return cells * size_t(Q);
+}
+
 hipError_t unpack_populations(int lattice, int storage, const void* f, double* out,
-                              const Geometry& g, int phase, hipStream_t s) {
+                              const Geometry& g, int phase, hipStream_t s, int layout) {
+  const bool aos = layout == LBM_LAYOUT_REF_AOS_GHOST;
   switch (lattice) {
-    case LBM_D2Q9: return unpack_l<D2Q9>(storage, f, out, g, phase, s);
-    case LBM_D3Q19: return unpack_l<D3Q19>(storage, f, out, g, phase, s);
-    case LBM_D3Q27: return unpack_l<D3Q27>(storage, f, out, g, phase, s);
+    case LBM_D2Q9: return aos ? unpack_l<D2Q9, true>(storage, f, out, g, phase, s) : unpack_l<D2Q9, false>(storage, f, out, g, phase, s);
+    case LBM_D3Q19: return aos ? unpack_l<D3Q19, true>(storage, f, out, g, phase, s) : unpack_l<D3Q19, false>(storage, f, out, g, phase, s);
+    case LBM_D3Q27: return aos ? unpack_l<D3Q27, true>(storage, f, out, g, phase, s) : unpack_l<D3Q27, false>(storage, f, out, g, phase, s);
   }
   return hipErrorInvalidValue;
 }
 hipError_t pack_populations(int lattice, int storage, void* f, const double* in, const Geometry& g,
-                            hipStream_t s) {
+                            hipStream_t s, int layout) {
+  const bool aos = layout == LBM_LAYOUT_REF_AOS_GHOST;
   switch (lattice) {
-    case LBM_D2Q9: return pack_l<D2Q9>(storage, f, in, g, s);
-    case LBM_D3Q19: return pack_l<D3Q19>(storage, f, in, g, s);
-    case LBM_D3Q27: return pack_l<D3Q27>(storage, f, in, g, s);
+    case LBM_D2Q9: return aos ? pack_l<D2Q9, true>(storage, f, in, g, s) : pack_l<D2Q9, false>(storage, f, in, g, s);
+    case LBM_D3Q19: return aos ? pack_l<D3Q19, true>(storage, f, in, g, s) : pack_l<D3Q19, false>(storage, f, in, g, s);
+    case LBM_D3Q27: return aos ? pack_l<D3Q27, true>(storage, f, in, g, s) : pack_l<D3Q27, false>(storage, f, in, g, s);
   }
   return hipErrorInvalidValue;
 }

@@ -16,10 +16,13 @@ struct Geometry {
   uint32_t periodic;
 };
 
+// host-side population arrays in `layout` (lbm_layout): LBM_LAYOUT_SOA or the
+// reference's chunk layout with its ghost ring (LBM_LAYOUT_REF_AOS_GHOST)
+size_t population_count(int lattice, const Geometry& g, int layout);
 hipError_t unpack_populations(int lattice, int storage, const void* f, double* out,
-                              const Geometry& g, int phase, hipStream_t s);
+                              const Geometry& g, int phase, hipStream_t s, int layout = LBM_LAYOUT_SOA);
 hipError_t pack_populations(int lattice, int storage, void* f, const double* in,
-                            const Geometry& g, hipStream_t s);
+                            const Geometry& g, hipStream_t s, int layout = LBM_LAYOUT_SOA);
 hipError_t field_to_double(bool is_double, const void* a, double* b, size_t n, hipStream_t s);
 hipError_t field_from_double(bool is_double, void* a, const double* b, size_t n, hipStream_t s);
 

@@ -29,7 +29,14 @@ struct StepArgs {
   uint64_t t;
   const uint64_t* t_dev;  // null: t is the step index; else the index is *t_dev + t
   int32_t write_macros;
+  // fault words the host reads without a copy (pinned, mapped host memory):
+  // [kStatusNonFinite] set to 1 when a macro-writing pass meets a non-finite
+  // density or velocity; [kStatusOverflow] when a cell-list bin was full
+  uint32_t* status;
 };
+
+constexpr int kStatusNonFinite = 0;
+constexpr int kStatusOverflow = 1;
 
 struct LaunchShape {
   int bx = 256, by = 1;  // threads per block along x and y (scalar kernel)
@@ -81,6 +88,22 @@ hipError_t launch_batch_residual(int dims, int storage, const BatchArgs& b, cons
                                  double* partial, const uint32_t* tile_begin, uint32_t members,
                                  double* out, hipStream_t s);
 
+// ---- residual sums (residual.hip) ---------------------------------------------
+// FP64 Σ|u − u_prev|² and Σ|u|² of one lattice in a fixed order: one partial
+// pair per block of kResidualCellsPerBlock cells, then out[0..2] = (r, Σ|Δu|²,
+// Σ|u|²) with r = sqrt(Σ|Δu|²/Σ|u|²) (0 when Σ|u|² = 0, NaN when a sum is not
+// finite).  partial holds 2 * residual_blocks(N) doubles.
+constexpr uint32_t kResidualCellsPerBlock = 2048;
+inline uint32_t residual_blocks(uint32_t N) {
+  return (N + kResidualCellsPerBlock - 1) / kResidualCellsPerBlock;
+}
+hipError_t launch_residual(int dims, bool dbl, const void* u, const void* up, uint32_t N, double* partial,
+                           double* out, hipStream_t s);
+// out[2m..2m+1] = the partial pairs [begin[m], begin[m+1]) summed in a fixed
+// order, one block per segment
+hipError_t launch_segment_sums(const double* partial, const uint32_t* begin, uint32_t segments, double* out,
+                               hipStream_t s);
+
 // Particle coupling (particles.hip)
 struct ParticleArgs {
   // lattice and fields
@@ -127,6 +150,7 @@ struct ParticleArgs {
   void* snap;             // n records: the particles as the gather saw them
   int32_t fixed_axis;  // ParticleAxisIntegrate instead of ParticleIntegrate
   double axis[3];      // its normalised axis, T-rounded (2D: the sign in [0])
+  uint32_t* status;    // the context's fault words (StepArgs::status)
 };
 
 // Up to this many particles the gather decides cell ownership per block (no

@@ -108,6 +108,15 @@ struct lbm_ctx {
   double bin = 1.0;
   uint64_t grid[3] = {1, 1, 1};
   uint64_t bins = 1;
+  // fault words written by the kernels (kStatusNonFinite, kStatusOverflow):
+  // pinned host memory, read by the host without a copy
+  uint32_t* status_h = nullptr;
+  uint32_t* status_d = nullptr;
+  // residual tracking: the velocity of the step before the last, the block
+  // partial sums and (r, Σ|Δu|², Σ|u|²)
+  void* u_prev = nullptr;
+  double* res_partial = nullptr;
+  double* res_out = nullptr;
   size_t tsz() const { return compute_double(storage) ? 8 : 4; }
 };
 
@@ -261,6 +270,7 @@ ParticleArgs particle_args(lbm_ctx* c) {
   a.umac = c->u;
   // fixed ramp width of the HLBM gather and the FPLBM particle (hlbm.hpp:173, fplbm.hpp:193)
   if (a.hlbm || a.fplbm) a.eps = to_T(pst, 1.5);
+  a.status = c->status_d;
   a.fixed_axis = c->pp.fixed_axis ? 1 : 0;
   if (pst == LBM_F64)
     spin_axis_normalised<double>(c->D, c->pp.spin_axis, a.axis);
@@ -316,14 +326,33 @@ StepArgs step_args(const lbm_ctx* c, uint64_t t, bool write_macros, const uint64
   a.t = t;
   a.t_dev = t_dev;
   a.write_macros = write_macros ? 1 : 0;
+  a.status = c->status_d;
   return a;
 }
 
-lbm_status fluid_pass(lbm_ctx* c, uint64_t t, bool write_macros, const uint64_t* t_dev = nullptr) {
-  const StepArgs a = step_args(c, t, write_macros, t_dev);
+// u_into: where this pass writes the velocity when it writes macros (null:
+// the macro field)
+lbm_status fluid_pass(lbm_ctx* c, uint64_t t, bool write_macros, const uint64_t* t_dev = nullptr,
+                      void* u_into = nullptr) {
+  StepArgs a = step_args(c, t, write_macros, t_dev);
+  if (u_into) a.u = u_into;
   HIP_TRY(launch_aa_step(c->lattice, c->storage, c->phase == 1, a, c->shape, c->stream));
   c->phase ^= 1;
   return LBM_OK;
+}
+
+// Faults the kernels recorded since the last report, reported once.
+lbm_status report_faults(lbm_ctx* c) {
+  if (!c->status_h) return LBM_OK;
+  volatile uint32_t* w = c->status_h;
+  const uint32_t nonfinite = w[kStatusNonFinite], overflow = w[kStatusOverflow];
+  if (!nonfinite && !overflow) return LBM_OK;
+  w[kStatusNonFinite] = 0;
+  w[kStatusOverflow] = 0;
+  if (nonfinite)
+    return fail(LBM_ERR_INVALID_STATE, "a macro-writing step met a non-finite density or velocity%s",
+                overflow ? " (and a cell-list bin was full)" : "");
+  return fail(LBM_ERR_OVERFLOW, "a cell-list bin was full (lbm_cell_list_overflow has the count)");
 }
 
 template <class X>
@@ -429,6 +458,11 @@ lbm_status lbm_ctx_create(lbm_ctx** out, int32_t lattice, int32_t storage,
     return bail(fail(LBM_ERR_CRITICAL, "hipStreamCreate"));
   if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
     return bail(fail(LBM_ERR_CRITICAL, "hipEventCreate"));
+  if (hipHostMalloc((void**)&c->status_h, 64, hipHostMallocMapped) != hipSuccess)
+    return bail(fail(LBM_ERR_OUT_OF_MEMORY, "fault words (pinned host memory)"));
+  std::memset(c->status_h, 0, 64);
+  if (hipHostGetDevicePointer((void**)&c->status_d, c->status_h, 0) != hipSuccess)
+    return bail(fail(LBM_ERR_CRITICAL, "hipHostGetDevicePointer"));
   hipError_t e = alloc_zero(&c->f, size_t(Q) * Np * storage_bytes(storage), c->stream);
   if (e == hipSuccess) e = alloc_zero((void**)&c->flags, N, c->stream);
   if (e == hipSuccess) e = alloc_zero((void**)&c->ops, sizeof(OpDev) * kMaxOps, c->stream);
@@ -457,9 +491,10 @@ lbm_status lbm_ctx_destroy(lbm_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   void* ptrs[] = {c->f, c->flags, c->ops, c->rho, c->u, c->force, c->por, c->bcvel, c->pn, c->pd,
                   c->parts, c->snap, c->f64, c->t64, c->partial, c->cl_count, c->cl_slot,
-                  c->cl_overflow, c->owner, c->t_dev};
+                  c->cl_overflow, c->owner, c->t_dev, c->u_prev, c->res_partial, c->res_out};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  if (c->status_h) (void)hipHostFree(c->status_h);
   if (c->step_graph) (void)hipGraphExecDestroy(c->step_graph);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -626,31 +661,53 @@ lbm_status lbm_get_porosity(lbm_ctx* c, double* p) {
   return get_field(c, c->por, p, c->N);
 }
 
-lbm_status lbm_set_populations(lbm_ctx* c, const double* f) {
+lbm_status lbm_population_count(lbm_ctx* c, int32_t layout, uint64_t* count) {
+  if (!c || !count) return fail(LBM_ERR_INVALID_ARG, "null argument");
+  if (layout != LBM_LAYOUT_SOA && layout != LBM_LAYOUT_REF_AOS_GHOST)
+    return fail(LBM_ERR_INVALID_ARG, "unknown population layout %d", layout);
+  *count = population_count(c->lattice, geom(c), layout);
+  return LBM_OK;
+}
+
+lbm_status lbm_set_populations_layout(lbm_ctx* c, const double* f, int32_t layout) {
   if (!c || !f) return fail(LBM_ERR_INVALID_ARG, "null argument");
-  const size_t cnt = size_t(c->Q) * c->N;
+  if (layout != LBM_LAYOUT_SOA && layout != LBM_LAYOUT_REF_AOS_GHOST)
+    return fail(LBM_ERR_INVALID_ARG, "unknown population layout %d", layout);
+  const size_t cnt = population_count(c->lattice, geom(c), layout);
   double* tmp = nullptr;
   HIP_TRY(hipSetDevice(c->device));
   HIP_TRY(hipMalloc(&tmp, cnt * sizeof(double)));
   HIP_TRY(hipMemcpyAsync(tmp, f, cnt * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  HIP_TRY(pack_populations(c->lattice, c->storage, c->f, tmp, geom(c), c->stream));
+  HIP_TRY(pack_populations(c->lattice, c->storage, c->f, tmp, geom(c), c->stream, layout));
   HIP_TRY(hipStreamSynchronize(c->stream));
   HIP_TRY(hipFree(tmp));
   c->phase = 0;
   return LBM_OK;
 }
 
-lbm_status lbm_get_populations(lbm_ctx* c, double* f) {
+lbm_status lbm_get_populations_layout(lbm_ctx* c, double* f, int32_t layout) {
   if (!c || !f) return fail(LBM_ERR_INVALID_ARG, "null argument");
-  const size_t cnt = size_t(c->Q) * c->N;
+  if (layout != LBM_LAYOUT_SOA && layout != LBM_LAYOUT_REF_AOS_GHOST)
+    return fail(LBM_ERR_INVALID_ARG, "unknown population layout %d", layout);
+  const size_t cnt = population_count(c->lattice, geom(c), layout);
   double* tmp = nullptr;
   HIP_TRY(hipSetDevice(c->device));
   HIP_TRY(hipMalloc(&tmp, cnt * sizeof(double)));
-  HIP_TRY(unpack_populations(c->lattice, c->storage, c->f, tmp, geom(c), c->phase, c->stream));
+  if (layout == LBM_LAYOUT_REF_AOS_GHOST)  // the ghost ring holds zeros
+    HIP_TRY(hipMemsetAsync(tmp, 0, cnt * sizeof(double), c->stream));
+  HIP_TRY(unpack_populations(c->lattice, c->storage, c->f, tmp, geom(c), c->phase, c->stream, layout));
   HIP_TRY(hipMemcpyAsync(f, tmp, cnt * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   HIP_TRY(hipFree(tmp));
   return LBM_OK;
+}
+
+lbm_status lbm_set_populations(lbm_ctx* c, const double* f) {
+  return lbm_set_populations_layout(c, f, LBM_LAYOUT_SOA);
+}
+
+lbm_status lbm_get_populations(lbm_ctx* c, double* f) {
+  return lbm_get_populations_layout(c, f, LBM_LAYOUT_SOA);
 }
 
 }  // extern "C"
@@ -744,17 +801,24 @@ lbm_status lbm_step(lbm_ctx* c, uint64_t t0, uint64_t nsteps) {
   if (!c) return fail(LBM_ERR_INVALID_ARG, "null ctx");
   lbm_status s = check_ready(c);
   if (s) return s;
+  s = report_faults(c);  // of earlier calls' work that has run
+  if (s) return s;
   HIP_TRY(hipSetDevice(c->device));
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
   // FPLBM reads the stored macro velocity of non-fluid cells every step
   const bool every = c->np > 0 && c->pp.coupling == LBM_COUPLING_FPLBM;
+  // residual tracking: the step before the last keeps its velocity in u_prev
+  const bool track = c->u_prev != nullptr;
+  const size_t ubytes = size_t(c->D) * c->N * c->tsz();
+  if (track && nsteps == 1)  // the step before is the previous call's last
+    HIP_TRY(hipMemcpyAsync(c->u_prev, c->u, ubytes, hipMemcpyDeviceToDevice, c->stream));
   const uint64_t G = c->steps_per_graph;
   uint64_t k = 0;
   while (k < nsteps) {
     const uint64_t t = t0 + k;
     // a whole graph of steps: from phase 0, and never the last step of the
-    // call (it writes the macros)
-    if (G > 0 && c->phase == 0 && nsteps - k > G) {
+    // call (it writes the macros), nor the one before it when tracking
+    if (G > 0 && c->phase == 0 && nsteps - k > G + (track ? 1u : 0u)) {
       s = replay_steps(c, t, every);
       if (s) return s;
       k += G;
@@ -764,7 +828,14 @@ lbm_status lbm_step(lbm_ctx* c, uint64_t t0, uint64_t nsteps) {
       s = particle_pass(c, t);
       if (s) return s;
     }
-    s = fluid_pass(c, t, k + 1 == nsteps || every);
+    const bool before_last = track && k + 2 == nsteps;
+    if (before_last && !every) {
+      s = fluid_pass(c, t, true, nullptr, c->u_prev);
+    } else {
+      s = fluid_pass(c, t, k + 1 == nsteps || every);
+      if (!s && before_last)
+        HIP_TRY(hipMemcpyAsync(c->u_prev, c->u, ubytes, hipMemcpyDeviceToDevice, c->stream));
+    }
     if (s) return s;
     ++k;
   }
@@ -776,6 +847,37 @@ lbm_status lbm_step(lbm_ctx* c, uint64_t t0, uint64_t nsteps) {
 lbm_status lbm_sync(lbm_ctx* c) {
   if (!c) return fail(LBM_ERR_INVALID_ARG, "null ctx");
   HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return report_faults(c);
+}
+
+lbm_status lbm_set_residual(lbm_ctx* c, int32_t enable) {
+  if (!c) return fail(LBM_ERR_INVALID_ARG, "null ctx");
+  HIP_TRY(hipSetDevice(c->device));
+  if (!enable) {
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    void* ptrs[] = {c->u_prev, c->res_partial, c->res_out};
+    for (void* p : ptrs)
+      if (p) HIP_TRY(hipFree(p));
+    c->u_prev = nullptr;
+    c->res_partial = c->res_out = nullptr;
+    return LBM_OK;
+  }
+  if (c->u_prev) return LBM_OK;
+  HIP_TRY(alloc_zero(&c->u_prev, size_t(c->D) * c->N * c->tsz(), c->stream));
+  HIP_TRY(alloc_zero((void**)&c->res_partial, 2 * size_t(residual_blocks(c->N)) * sizeof(double), c->stream));
+  HIP_TRY(alloc_zero((void**)&c->res_out, 3 * sizeof(double), c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return LBM_OK;
+}
+
+lbm_status lbm_residual(lbm_ctx* c, double out[3]) {
+  if (!c || !out) return fail(LBM_ERR_INVALID_ARG, "null argument");
+  if (!c->u_prev) return fail(LBM_ERR_INVALID_STATE, "residual tracking is off (lbm_set_residual)");
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(launch_residual(c->D, compute_double(c->storage), c->u, c->u_prev, c->N, c->res_partial, c->res_out,
+                          c->stream));
+  HIP_TRY(hipMemcpyAsync(out, c->res_out, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   return LBM_OK;
 }
@@ -1215,6 +1317,7 @@ StepArgs member_args(const lbm_ctx* c) {
   a.N = c->N;
   a.Np = c->Np;
   a.periodic = c->periodic;
+  a.status = c->status_d;
   return a;
 }
 

@@ -607,6 +607,7 @@ __global__ __launch_bounds__(1024) void k_build_cells(ParticleArgs a, uint64_t b
       atomicMax((unsigned long long*)(a.cl_count + bp), (unsigned long long)(rank + 1));
     } else {
       atomicAdd((unsigned long long*)a.cl_overflow, 1ull);
+      if (a.status) __hip_atomic_store(a.status + kStatusOverflow, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
   }
 }

@@ -183,8 +183,16 @@ __device__ __forceinline__ void apply_op(T (&f)[L::Q], OpP op, const StepArgs& a
       if (a.write_macros) {
         gfield<T>(a.rho)[cell] = rho;
         auto um = gfield<T>(a.u);
+        bool bad = !__builtin_isfinite(rho);
 #pragma unroll
-        for (int d = 0; d < L::D; ++d) um[size_t(d) * a.N + cell] = u[d];
+        for (int d = 0; d < L::D; ++d) {
+          um[size_t(d) * a.N + cell] = u[d];
+          bad = bad || !__builtin_isfinite(u[d]);
+        }
+        // one store per wavefront that meets a non-finite value
+        const uint64_t m = __ballot(bad);
+        if (m != 0 && a.status && __lane_id() == uint32_t(__ffsll((unsigned long long)m) - 1))
+          __hip_atomic_store(a.status + kStatusNonFinite, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
       }
       const T om = T(op->omega);
       if (coll == LBM_COLL_BGK) {
@@ -890,19 +898,6 @@ __global__ __launch_bounds__(256) void k_batch_residual(BatchArgs b, double* par
   }
 }
 
-__global__ void k_batch_residual_sum(const double* partial, const uint32_t* tile_begin,
-                                     uint32_t members, double* out) {
-  const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= members) return;
-  double d = 0.0, u = 0.0;
-  for (uint32_t k = tile_begin[m]; k < tile_begin[m + 1]; ++k) {
-    d += partial[size_t(k) * 2];
-    u += partial[size_t(k) * 2 + 1];
-  }
-  out[2 * m] = d;
-  out[2 * m + 1] = u;
-}
-
 template <class L, class S, class T, bool WB>
 static hipError_t launch_batch_t(bool odd, const BatchArgs& b, const BatchShape& sh, hipStream_t s) {
   const uint32_t grid = sh.grid > 0 ? uint32_t(sh.grid) : b.ntiles;
@@ -949,9 +944,10 @@ hipError_t launch_batch_residual(int dims, int storage, const BatchArgs& b, cons
     if (dbl) hipLaunchKernelGGL((k_batch_residual<double, 3>), dim3(grid), dim3(sh.block), 0, s, b, partial);
     else hipLaunchKernelGGL((k_batch_residual<float, 3>), dim3(grid), dim3(sh.block), 0, s, b, partial);
   }
-  hipLaunchKernelGGL(k_batch_residual_sum, dim3((members + 255) / 256), dim3(256), 0, s, partial,
-                     tile_begin, members, out);
-  return hipGetLastError();
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  // per member, its tiles' partial sums in a fixed order (one block each)
+  return launch_segment_sums(partial, tile_begin, members, out, s);
 }
 
 template <class L, class S, class T, bool WB>

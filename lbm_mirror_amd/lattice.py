@@ -101,16 +101,25 @@ class Lattice:
         check(self.L.lbm_get_hlbm_sums(self._h, _dp(n), _dp(d)))
         return n.reshape(self.D, self.N), d
 
-    def set_populations(self, f):
-        a = np.ascontiguousarray(f, dtype=np.float64).reshape(-1)
-        if a.size != self.Q * self.N:
-            raise ValueError("populations: expected Q*N values")
-        check(self.L.lbm_set_populations(self._h, _dp(a)))
+    def population_count(self, layout=abi.LAYOUT_SOA):
+        n = C.c_uint64()
+        check(self.L.lbm_population_count(self._h, layout, C.byref(n)))
+        return n.value
 
-    def get_populations(self):
-        a = np.zeros(self.Q * self.N)
-        check(self.L.lbm_get_populations(self._h, _dp(a)))
-        return a.reshape(self.Q, self.N)
+    def set_populations(self, f, layout=abi.LAYOUT_SOA):
+        """SoA f[i, cell] (default) or the reference's chunk layout with its
+        ghost ring, flat (lbm_set_populations_layout)."""
+        a = np.ascontiguousarray(f, dtype=np.float64).reshape(-1)
+        if a.size != self.population_count(layout):
+            raise ValueError(f"populations: expected {self.population_count(layout)} values")
+        check(self.L.lbm_set_populations_layout(self._h, _dp(a), layout))
+
+    def get_populations(self, layout=abi.LAYOUT_SOA):
+        """(Q, N) SoA by default; the chunk layout as a flat array of
+        (ghost-widened cells) x Q doubles."""
+        a = np.zeros(self.population_count(layout))
+        check(self.L.lbm_get_populations_layout(self._h, _dp(a), layout))
+        return a.reshape(self.Q, self.N) if layout == abi.LAYOUT_SOA else a
 
     def init_equilibrium(self, rho=1.0, u=(0.0, 0.0, 0.0)):
         uu = np.zeros(3)
@@ -136,6 +145,15 @@ class Lattice:
 
     def sync(self):
         check(self.L.lbm_sync(self._h))
+
+    def set_residual(self, enable=True):
+        check(self.L.lbm_set_residual(self._h, 1 if enable else 0))
+
+    def residual(self):
+        """(r, sum |u_t - u_{t-1}|^2, sum |u_t|^2) in FP64 (lbm_residual)."""
+        out = np.zeros(3)
+        check(self.L.lbm_residual(self._h, _dp(out)))
+        return float(out[0]), float(out[1]), float(out[2])
 
     def set_step_graph(self, steps):
         """Steps per hipGraph replay inside step() (0: launch every step)."""
