@@ -270,6 +270,20 @@ lbm_status lbm_sync(lbm_ctx* ctx);
  * allocates (enable = 1) or frees (0) the second velocity field. */
 lbm_status lbm_set_residual(lbm_ctx* ctx, int32_t enable);
 lbm_status lbm_residual(lbm_ctx* ctx, double out[3]);
+/* lbm_residual without allocating anything.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid it).
+ * The same kernels and summation order as lbm_residual (equal bits), with
+ * the block sums in a static device array of the library (8192 pairs of
+ * doubles, 128 KiB per GPU, held until the process ends) instead of the
+ * context's buffer; `out_device` is caller-owned device memory for the 3
+ * doubles, written asynchronously on the context's stream (visible after
+ * lbm_sync or any wait on that stream; no copy, no wait here).  Lattices
+ * over 16,777,216 cells are rejected with LBM_ERR_NOT_SUPPORTED.  All calls
+ * on one GPU share that array and every call rewrites all it reads, so
+ * consecutive calls need nothing cleared; calls that overlap on one GPU
+ * (contexts on other streams, host threads) race. */
+lbm_status lbm_residual_noalloc(lbm_ctx* ctx, double* out_device);
 /* Device time of the last lbm_step call, from HIP events recorded on the
  * context's stream around its launches. */
 lbm_status lbm_last_step_ms(lbm_ctx* ctx, double* ms);

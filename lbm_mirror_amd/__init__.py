@@ -47,7 +47,7 @@ EXPORTS = [
     "lbm_read_forst_fields", "lbm_set_step_graph", "lbm_hbm_ceiling",
     "lbm_autotune", "lbm_set_step_tuning", "lbm_get_step_tuning",
     "lbm_population_count", "lbm_set_populations_layout", "lbm_get_populations_layout",
-    "lbm_set_residual", "lbm_residual",
+    "lbm_set_residual", "lbm_residual", "lbm_residual_noalloc",
 ]
 
 
@@ -133,6 +133,7 @@ def load_library(path=None):
     L.lbm_get_populations_layout.argtypes = [vp, dp, C.c_int32]
     L.lbm_set_residual.argtypes = [vp, C.c_int32]
     L.lbm_residual.argtypes = [vp, dp]
+    L.lbm_residual_noalloc.argtypes = [vp, vp]
     L.lbm_ctx_info.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), u64p]
     cs = C.c_char_p
     L.lbm_write_vtk.argtypes = [vp, cs, cs, C.c_uint64, cs]

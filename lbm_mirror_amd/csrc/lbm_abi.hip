@@ -12,6 +12,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -52,6 +53,31 @@ bool compute_double(int st) { return st == LBM_F64; }
 double to_T(int st, double v) { return compute_double(st) ? v : double(float(v)); }
 
 const int kMaxParticles = 4096;
+
+// ---- lbm_residual_noalloc's scratch ------------------------------------------
+// The block sums of the residual (2 doubles per block of
+// kResidualCellsPerBlock cells) for callers that cannot allocate: a file-scope
+// device array, one copy per GPU, held until the process ends.  Host code
+// never passes `scratch` itself to a kernel (in host code the name is a
+// host-side stand-in): each GPU's address is looked up once, with that GPU
+// current, when the first context on it is created.
+constexpr uint32_t kResidualScratchBlocks = 8192;  // lattices up to 16,777,216 cells
+constexpr int kMaxDevices = 64;
+__device__ __align__(256) double scratch[2 * kResidualScratchBlocks];
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// the current device's copy of `scratch`; call with `device` current
+hipError_t scratch_lookup(int device) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return hipSuccess;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  g_scratch_ptr[device] = static_cast<double*>(p);
+  return hipSuccess;
+}
 
 }  // namespace
 
@@ -454,6 +480,11 @@ lbm_status lbm_ctx_create(lbm_ctx** out, int32_t lattice, int32_t storage,
     return s;
   };
   if (hipSetDevice(device) != hipSuccess) return bail(fail(LBM_ERR_CRITICAL, "hipSetDevice"));
+  {
+    const hipError_t e = scratch_lookup(device);
+    if (e != hipSuccess)
+      return bail(fail(LBM_ERR_CRITICAL, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e)));
+  }
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
     return bail(fail(LBM_ERR_CRITICAL, "hipStreamCreate"));
   if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
@@ -879,6 +910,23 @@ lbm_status lbm_residual(lbm_ctx* c, double out[3]) {
                           c->stream));
   HIP_TRY(hipMemcpyAsync(out, c->res_out, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
+  return LBM_OK;
+}
+
+lbm_status lbm_residual_noalloc(lbm_ctx* c, double* out_device) {
+  if (!c || !out_device) return fail(LBM_ERR_INVALID_ARG, "null argument");
+  if (residual_blocks(c->N) > kResidualScratchBlocks)
+    return fail(LBM_ERR_NOT_SUPPORTED,
+                "lbm_residual_noalloc: %u cells exceed its %u-cell maximum (scratch of %u block sums); "
+                "use lbm_residual",
+                c->N, kResidualScratchBlocks * kResidualCellsPerBlock, kResidualScratchBlocks);
+  if (!c->u_prev) return fail(LBM_ERR_INVALID_STATE, "residual tracking is off (lbm_set_residual)");
+  double* partial = g_scratch_ptr[c->device];  // looked up when the context was created
+  if (!partial) return fail(LBM_ERR_INVALID_STATE, "no scratch address for device %d", c->device);
+  HIP_TRY(hipSetDevice(c->device));
+  // every block sum the final pass reads is written first: nothing to clear
+  HIP_TRY(launch_residual(c->D, compute_double(c->storage), c->u, c->u_prev, c->N, partial, out_device,
+                          c->stream));
   return LBM_OK;
 }
 
