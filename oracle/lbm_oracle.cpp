@@ -43,6 +43,16 @@
 // that C++ promotes them the same way (equilibrium bracket, DEM damping,
 // contact point, disc inertia).  Build with -ffp-contract=off.
 //
+// Two switches depart from the reference on purpose, off by default, for
+// comparing the GPU at tighter tolerances than the reference's own
+// arithmetic allows: SimBase::sum64 applies the particle force/torque sums
+// as FP64 sums rounded to T once (the GPU's arithmetic) instead of the
+// reference's serial T sums (in FP64 the same sums in the same order, a CPU
+// test holds them bitwise equal; the FP64 sums are pinned by brute force);
+// SimBase::series_sin takes the porosity ramp's sine from the engine's
+// series instead of the C library (pinned to one ulp of the correctly
+// rounded sine, and equal to it after rounding to float, by a CPU test).
+//
 // Layout is the reference's: two AoS population buffers ("dfs_old", "dfs")
 // with a one-cell ghost ring that stays zero (chunk.hpp:25-35), selected by
 // time-step parity (collision.hpp:61-64, stream.hpp:28-31).
@@ -148,9 +158,33 @@ struct Part {
   T mass, inertia, density, radius;
 };
 
+// sin on [0, pi/2] by series (x + x^3 P(x^2) up to x^17 below pi/4,
+// cos(pi/2 - x) = 1 + z Q(z) up to z^8 above, pi/2 - x with pi/2 in two
+// doubles; coefficients the doubles nearest (-1)^k/n!).  The engine's own
+// sine for the porosity ramp; used here only where SimBase::series_sin is
+// set, to compare the GPU's porosity bit for bit.  Within one ulp of the
+// correctly rounded sin on the ramp's range (tests/test_oracle.py).
+inline double series_sin_q1(double x) {
+  static const double kS[8] = {-1.0 / 6.0, 1.0 / 120.0, -1.0 / 5040.0, 1.0 / 362880.0, -1.0 / 39916800.0,
+                               1.0 / 6227020800.0, -1.0 / 1307674368000.0, 1.0 / 355687428096000.0};
+  static const double kC[8] = {-1.0 / 2.0, 1.0 / 24.0, -1.0 / 720.0, 1.0 / 40320.0, -1.0 / 3628800.0,
+                               1.0 / 479001600.0, -1.0 / 87178291200.0, 1.0 / 20922789888000.0};
+  if (x > 0.78539816339744831) {
+    const double r = (1.5707963267948966 - x) + 6.123233995736766e-17;
+    const double z = r * r;
+    double q = kC[7];
+    for (int k = 6; k >= 0; --k) q = kC[k] + z * q;
+    return 1.0 + z * q;
+  }
+  const double z = x * x;
+  double p = kS[7];
+  for (int k = 6; k >= 0; --k) p = kS[k] + z * p;
+  return x + x * (z * p);
+}
+
 // particle_porosity<..., ParticleSpheroid>::calculate (porosity.hpp:56-89)
 template <class T, int D>
-T porosity(const T* rel, T rad, T eps) {
+T porosity(const T* rel, T rad, T eps, bool series_sin = false) {
   T s2 = T(0);
   for (int d = 0; d < D; ++d) s2 = T(s2 + T(rel[d] * rel[d]));
   T s = std::sqrt(s2);
@@ -159,7 +193,7 @@ T porosity(const T* rel, T rad, T eps) {
   if (s <= lo) return T(0);
   if (s >= hi) return T(1);
   T inner = (M_PI / (2 * eps)) * T(s - lo);
-  T si = std::sin(inner);
+  T si = series_sin ? T(series_sin_q1(static_cast<double>(inner))) : std::sin(inner);
   return T(si * si);
 }
 
@@ -539,6 +573,16 @@ struct CellList {
 
 // ------------------------------------------------------------ simulation
 struct SimBase {
+  // Particle force and torque sums: false (default) the reference's serial
+  // sums in T (psm.hpp:404-443, hlbm.hpp:234,249-252, fplbm.hpp:224-236);
+  // true the FP64 sums of the same terms, rounded to T once (the GPU's
+  // arithmetic, which sums in FP64 by a tree; the order of an FP64 sum
+  // changes its last bits only).
+  bool sum64 = false;
+  // The porosity ramp's sine: false (default) the C library's std::sin, as
+  // the reference calls it (porosity.hpp:82); true the engine's series
+  // (series_sin_q1), so the GPU's porosity can be compared bit for bit.
+  bool series_sin = false;
   virtual ~SimBase() {}
   virtual int set_flags(const uint8_t*) = 0;
   virtual void set_periodic(uint32_t) = 0;
@@ -894,7 +938,7 @@ struct Sim final : SimBase {
         if (op.flags & LBM_OPF_PSM_STAMP_DISC) {
           T rel[3];
           for (int d = 0; d < D; ++d) rel[d] = T(static_cast<T>(x[d]) - T(op.disc[d]));
-          por[c] = porosity<T, D>(rel, T(op.disc[3]), T(op.disc[4]));
+          por[c] = porosity<T, D>(rel, T(op.disc[3]), T(op.disc[4]), this->series_sin);
         }
         T r, u[3], eq[27];
         rho_u<T, L>(f, r, u);
@@ -1010,7 +1054,7 @@ struct Sim final : SimBase {
           const uint64_t c = cell(x);
           T rel[3];
           for (int d = 0; d < D; ++d) rel[d] = T(static_cast<T>(x[d]) - P.pos[d]);
-          T B = porosity<T, D>(rel, P.radius, eps);
+          T B = porosity<T, D>(rel, P.radius, eps, this->series_sin);
           if (write_porosity) por[c] = B;
           if (B >= 1.0) continue;
           const T* f = a + gcell(x) * Q;
@@ -1039,7 +1083,7 @@ struct Sim final : SimBase {
           cross_rf<T, D>(rel, fc, tq);
           for (int d = 0; d < (D == 2 ? 1 : 3); ++d) t64[d] -= static_cast<double>(tq[d]);
         }
-    for (int d = 0; d < D; ++d) P.force[d] = T(P.force[d] + fp[d]);
+    for (int d = 0; d < D; ++d) P.force[d] = this->sum64 ? T(P.force[d] + T(f64[d])) : T(P.force[d] + fp[d]);
     if (pp.psm_torque) {
       for (int d = 0; d < (D == 2 ? 1 : 3); ++d) P.torque[d] = T(P.torque[d] + T(t64[d]));
     }
@@ -1072,7 +1116,7 @@ struct Sim final : SimBase {
           const uint64_t c = cell(x);
           T rel[3];
           for (int d = 0; d < D; ++d) rel[d] = T(static_cast<T>(x[d]) - P.pos[d]);
-          const T B = porosity<T, D>(rel, P.radius, eps);
+          const T B = porosity<T, D>(rel, P.radius, eps, this->series_sin);
           por[c] = B;
           if (B >= 1.0f) continue;
           const T* f = a + gcell(x) * Q;
@@ -1101,8 +1145,9 @@ struct Sim final : SimBase {
             t64[d] -= static_cast<double>(tq[d]);
           }
         }
-    for (int d = 0; d < D; ++d) P.force[d] = T(P.force[d] - fp[d]);
-    for (int d = 0; d < (D == 2 ? 1 : 3); ++d) P.torque[d] = T(P.torque[d] - tp[d]);
+    for (int d = 0; d < D; ++d) P.force[d] = this->sum64 ? T(P.force[d] + T(f64[d])) : T(P.force[d] - fp[d]);
+    for (int d = 0; d < (D == 2 ? 1 : 3); ++d)
+      P.torque[d] = this->sum64 ? T(P.torque[d] + T(t64[d])) : T(P.torque[d] - tp[d]);
     for (int d = 0; d < 3; ++d) {
       F64[p * 3 + d] = f64[d];
       T64[p * 3 + d] = t64[d];
@@ -1131,7 +1176,7 @@ struct Sim final : SimBase {
           const uint64_t c = cell(x);
           T rel[3];
           for (int d = 0; d < D; ++d) rel[d] = T(static_cast<T>(x[d]) - P.pos[d]);
-          const T B = porosity<T, D>(rel, P.radius, eps);
+          const T B = porosity<T, D>(rel, P.radius, eps, this->series_sin);
           por[c] = B;
           const T flip = T(one - B);
           T mom[3] = {T(0), T(0), T(0)};
@@ -1150,7 +1195,7 @@ struct Sim final : SimBase {
             abs64 += std::fabs(static_cast<double>(fc));
           }
         }
-    for (int d = 0; d < D; ++d) P.force[d] = fp[d];
+    for (int d = 0; d < D; ++d) P.force[d] = this->sum64 ? T(f64[d]) : fp[d];
     for (int d = 0; d < 3; ++d) {
       F64[p * 3 + d] = f64[d];
       T64[p * 3 + d] = 0.0;
@@ -1279,7 +1324,7 @@ struct Sim final : SimBase {
         cl.for_each_candidate(x, [&](uint64_t p) {
           T rel[3];
           for (int d = 0; d < D; ++d) rel[d] = T(cp[d] - parts[p].pos[d]);
-          B = T(B * porosity<T, D>(rel, parts[p].radius, eps));
+          B = T(B * porosity<T, D>(rel, parts[p].radius, eps, this->series_sin));
         });
         por[c] = B;
       }
@@ -1559,6 +1604,9 @@ void orc_dem_contact(void* s) { static_cast<SimBase*>(s)->dem_contact(); }
 void orc_particle_integrate(void* s) { static_cast<SimBase*>(s)->integrate_all(); }
 uint64_t orc_cell_list_overflow(void* s) { return static_cast<SimBase*>(s)->overflow(); }
 void orc_sim_get_hlbm(void* s, double* nsum, double* dsum) { static_cast<SimBase*>(s)->get_hlbm(nsum, dsum); }
+void orc_sim_set_sum64(void* s, int on) { static_cast<SimBase*>(s)->sum64 = on != 0; }
+void orc_sim_set_series_sin(void* s, int on) { static_cast<SimBase*>(s)->series_sin = on != 0; }
+double orc_series_sin(double x) { return series_sin_q1(x); }
 
 // narrow_phase<Spheroid>::contact on a canonical (lo, hi) pair.
 int orc_narrow(int D, int st, const double* pos_lo, double r_lo, const double* pos_hi, double r_hi,

@@ -62,6 +62,10 @@ def lib():
         L.orc_cell_hlbm.argtypes = [C.c_int, C.c_int, C.POINTER(abi.OpDesc), dp, C.c_double, dp, C.c_double,
                                     dp, dp]
         L.orc_sim_get_hlbm.argtypes = [vp, dp, dp]
+        L.orc_sim_set_sum64.argtypes = [vp, C.c_int]
+        L.orc_sim_set_series_sin.argtypes = [vp, C.c_int]
+        L.orc_series_sin.argtypes = [C.c_double]
+        L.orc_series_sin.restype = C.c_double
         L.orc_cell_list_overflow.restype = C.c_uint64
         L.orc_max_threads.restype = C.c_int
         L.orc_narrow.argtypes = [C.c_int, C.c_int, dp, C.c_double, dp, C.c_double, dp, dp, dp]
@@ -168,6 +172,11 @@ def narrow(D, st, pos_lo, r_lo, pos_hi, r_hi):
     pt = np.zeros(3)
     hit = lib().orc_narrow(D, st, _dp(a), r_lo, _dp(b), r_hi, _dp(depth), _dp(n), _dp(pt))
     return bool(hit), depth[0], n[:D], pt[:D]
+
+
+def series_sin(x):
+    """The engine's series sine of the porosity ramp (oracle restatement)."""
+    return lib().orc_series_sin(float(x))
 
 
 def max_threads():
@@ -284,6 +293,22 @@ class Sim:
 
     def overflow(self):
         return lib().orc_cell_list_overflow(self._h)
+
+    def set_particle_sums_fp64(self, on=True):
+        """Particle force/torque sums in FP64, rounded to T once (the GPU's
+        arithmetic), instead of the reference's serial T sums (default)."""
+        lib().orc_sim_set_sum64(self._h, 1 if on else 0)
+
+    def set_series_sin(self, on=True):
+        """The porosity ramp's sine from the engine's series instead of the C
+        library's sin (the reference's)."""
+        lib().orc_sim_set_series_sin(self._h, 1 if on else 0)
+
+    def match_gpu_arithmetic(self):
+        """Both switches the GPU comparison tests use: FP64 particle sums and
+        the engine's ramp sine."""
+        self.set_particle_sums_fp64(True)
+        self.set_series_sin(True)
 
     def get_hlbm(self):
         """HLBM particle sums: N (D x N) and D (N)."""

@@ -85,6 +85,14 @@ def test_batch_autotune_restores_state(hip):
     b.step(10)
     ms_, upd = b.stats()
     assert upd == 10 * sum(int(np.prod(m[0])) for m in ms) and ms_ > 0
+    # the timed trials replay captured graphs: a second tuning measures the
+    # same kind of time, and a chain of the chosen shape runs at about it
+    cfg2 = b.autotune()
+    assert 0.5 < cfg2["ms_per_step"] / cfg["ms_per_step"] < 2.0
+    b.set_tuning(cfg2["block"], cfg2["rows"], cfg2["grid"])
+    b.step(8)
+    b.step(8)  # the second call replays the first one's graph
+    assert b.stats()[0] / 8 < 3.0 * cfg2["ms_per_step"]
 
 
 def test_batch_run_reports_diverged_member_as_nan(hip):

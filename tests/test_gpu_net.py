@@ -64,6 +64,70 @@ def test_loss_gradient_matches_oracle(hip, H, W, widths, B):
         off += n
 
 
+def _dyadic_params(widths, rng):
+    """Sparse power-of-two weights: every output channel reads one input
+    channel (co -> co mod Cin, so an input channel feeds at most two outputs)
+    at one tap (all nine taps in use across channels), weight +-2^-k (mostly
+    positive, so activations stay alive through the ReLUs), bias a multiple of
+    1/16 in [-1/16, 1/4].  Every sum the forward and data-gradient passes form
+    then has at most two terms of short dyadic values and is exact in FP32
+    whatever the order, so the GPU and the oracle round to bf16 at the same
+    values; only the weight- and bias-gradient sums over pixels round (FP32)."""
+    layers = []
+    for li, (ci, co, _, _) in enumerate(no.layer_specs(widths)):
+        w = np.zeros((9, ci, co), np.float32)
+        for o in range(co):
+            w[(5 * o + li) % 9, o % ci, o] = rng.choice([-1.0, 1.0, 1.0, 1.0]) * 2.0 ** -int(rng.integers(0, 3))
+        b = (rng.integers(-1, 5, co) / 16.0).astype(np.float32)
+        layers.append((w, b))
+    return no.pack(layers).astype(np.float32)
+
+
+@pytest.mark.parametrize("H,W,widths,B", [(32, 32, (16, 16, 24, 32), 2), (64, 64, (32, 64, 96, 160), 2),
+                                          (24, 40, (8, 24, 40, 56), 3)])
+def test_gradient_per_tensor_with_exact_arithmetic(hip, H, W, widths, B):
+    """Each weight and bias gradient tensor on its own, where the arithmetic
+    leaves no room for order-dependent bf16 rounding: bf16-exact inputs,
+    dyadic sparse weights (_dyadic_params), targets one of +-1/4, +-1/2 off
+    the oracle's output and a mask over a power-of-two number of cells, so
+    the loss gradient is exact too.  The forward output is then bitwise the
+    oracle's, and every gradient tensor agrees to 1e-5 relative L2 (its FP32
+    pixel sums, accumulated in another order)."""
+    from lbm_mirror_amd.warmstart import Net
+    rng = np.random.default_rng(23)
+    net = Net(H, W, widths, batch=B, seed=12)
+    net.set_params(_dyadic_params(widths, rng))
+    params = net.get_params()
+    x = no.bf16(_x(rng, B, H, W))
+    y = net.forward_raw(x)
+    ref = no.forward(params, widths, x, mode="bf16")
+    assert np.array_equal(y, ref), f"forward differs, max {np.max(np.abs(y - ref)):.3e}"
+    t = (ref + rng.choice([-0.5, -0.25, 0.25, 0.5], ref.shape)).astype(np.float32)
+    m = np.zeros(B * H * W, np.float32)
+    count = 1 << int(np.log2(B * H * W // 2))  # a power of two
+    m[rng.permutation(B * H * W)[:count]] = 1
+    m = m.reshape(B, H, W)
+    loss, g = net.loss_grad_raw(x, t, m)
+    lref, gref = no.loss_grad(params, widths, x, t, m, mode="bf16")
+    assert abs(loss - lref) <= 1e-9 * lref
+    off, live = 0, 0
+    for i, (ci, co, _, _) in enumerate(no.layer_specs(widths)):
+        n = 9 * ci * co
+        for name, a, b in (("W", g[off:off + n], gref[off:off + n]),
+                           ("b", g[off + n:off + n + co], gref[off + n:off + n + co])):
+            nb = np.linalg.norm(b)
+            if nb == 0:  # one input per output channel: the narrow networks
+                assert np.all(a == 0), (i, name)  # leave some early layers unreached
+                continue
+            live += 1
+            err = np.linalg.norm(a - b) / nb
+            print(f"layer {i} {name}: grad rel L2 {err:.2e}")
+            assert err <= 1e-5, (i, name, err)
+        off += n + co
+    # the warm-start network's own widths reach every tensor
+    assert live == 24 if widths == (32, 64, 96, 160) else live >= 12
+
+
 def test_features_match_oracle(hip):
     from lbm_mirror_amd.warmstart import Net
     H, W = 16, 24
@@ -205,3 +269,16 @@ def test_trained_network_gain_and_warm_start(hip):
     # two runs stopped at residual r = 1e-6 differ by about r / lambda, lambda the
     # slowest decay rate per step (>= 1e-3 on these 64x64 channels)
     assert np.max(diffs) < 1e-3
+    # run on to a residual 1e-9 (or FP32's fixed point): the warm and cold
+    # starts then end at the same field to 1e-5 relative
+    tight = 1e-9
+    sc2, rc2 = cold.run(tol=tight, check_every=200, max_steps=200000, t0=int(sc.max()) + 2)
+    sw2, rw2 = warm.run(tol=tight, check_every=200, max_steps=200000, t0=int(sw.max()) + 2)
+    diffs2 = []
+    for i in np.flatnonzero(ok):
+        _, uc = cold.members[i].get_macros()
+        _, uw = warm.members[i].get_macros()
+        diffs2.append(np.linalg.norm(uw - uc) / np.linalg.norm(uc))
+    print(f"run on to 1e-9: residuals cold max {np.max(rc2[ok]):.1e}, warm max {np.max(rw2[ok]):.1e}; "
+          f"field difference max {np.max(diffs2):.1e}")
+    assert np.max(diffs2) < 1e-5

@@ -108,22 +108,23 @@ def _driver(orc, st, spin, zflags=0):
                                   coupling=abi.COUPLING_HLBM)
     for s in (g, o):
         s.particles_set([p], prm)
+    o.match_gpu_arithmetic()  # FP64 force/torque sums, the engine's ramp sine
     return g, o, p
 
 
 def _compare(g, o, st):
-    # FP64 force/torque sums on the GPU, serial T sums in the oracle
-    tol = 1e-11 if st == abi.F64 else 1e-4
+    # FP64 force/torque sums on both sides (tree on the GPU, serial in the oracle)
+    tol = 1e-11 if st == abi.F64 else 1e-5
     a, b = g.particles_get()[0], o.particles_get()[0]
     for d in range(3):
         assert abs(a.position[d] - b.position[d]) <= tol * max(abs(b.position[d]), 1.0)
         assert abs(a.rotation[d] - b.rotation[d]) <= tol * max(abs(b.rotation[d]), 1e-6) + 1e-15
     rg, ug = g.get_macros()
     ro, uo = o.get_macros()
-    ftol = 1e-12 if st == abi.F64 else 1e-4
+    ftol = 1e-12 if st == abi.F64 else 1e-5
     assert np.max(np.abs(rg - ro)) <= ftol
     assert np.max(np.abs(ug - uo)) <= ftol * np.max(np.abs(uo))
-    assert np.max(np.abs(g.get_porosity() - o.get_porosity())) <= (1e-13 if st == abi.F64 else 1e-6)
+    assert np.max(np.abs(g.get_porosity() - o.get_porosity())) <= (1e-13 if st == abi.F64 else 1e-5)
     return a, b
 
 

@@ -685,6 +685,56 @@ def test_psm_gather_brute_force_and_newton(orc):
     assert q.torque[0] == 0.0  # reference PSM gathers no torque (Q9)
 
 
+def _particle_channel(orc, st, coupling, sum64):
+    from helpers import seeded_discs
+    ext = (96, 48)
+    coll = {abi.COUPLING_PSM: abi.COLL_PSM, abi.COUPLING_HLBM: abi.COLL_HLBM,
+            abi.COUPLING_FPLBM: abi.COLL_FPLBM}[coupling]
+    s = orc.Sim(abi.D2Q9, st, ext)
+    s.set_flags(channel_flags_2d(*ext))
+    edge = abi.COLL_HLBM if coupling == abi.COUPLING_HLBM else abi.COLL_BGK
+    for fl, op in {1: abi.op_desc(boundary=abi.BC_BOUNCE_BACK), 2: abi.op_desc(collision=coll, tau=0.8),
+                   3: abi.op_desc(boundary=abi.BC_ZOUHE_PRESSURE_X, collision=edge, east=1, tau=0.8,
+                                  ramp=(1.0, 0.01 / 8, 1.01, 8)),
+                   4: abi.op_desc(boundary=abi.BC_ZOUHE_PRESSURE_X, collision=edge, east=0, tau=0.8,
+                                  rho=1.0)}.items():
+        s.set_operator(fl, op)
+    s.init_equilibrium(1.0, (0.0, 0.0))
+    discs = seeded_discs(6, ext, 4.0, 7.0, 21, margin=4.0, gap=4.0)
+    parts = [make_particle([x, y], r, 1.5, 2, st, orc=orc) for (x, y, r) in discs]
+    s.particles_set(parts, abi.particle_params(k_n=0.0, zeta_n=0.0, sub_steps=16, psm_torque=True,
+                                               body_acceleration=(0.0, -1e-4), max_radius=7.0,
+                                               coupling=coupling))
+    s.set_particle_sums_fp64(sum64)
+    return s
+
+
+@pytest.mark.parametrize("coupling", [abi.COUPLING_PSM, abi.COUPLING_HLBM, abi.COUPLING_FPLBM])
+def test_particle_sum64_mode(orc, coupling):
+    """The oracle's FP64-sum switch (the GPU's arithmetic): in FP64 it is the
+    same sums in the same order, so whole runs are bitwise those of the
+    reference's serial T sums; in FP32 each particle force is exactly the
+    FP64 sum (pinned by brute force above) rounded once, and stays within the
+    FP32 summation error of the serial sum."""
+    a, b = (_particle_channel(orc, F64, coupling, m) for m in (False, True))
+    a.step(20)
+    b.step(20)
+    assert np.array_equal(a.get_populations(), b.get_populations())
+    for p, q in zip(a.particles_get(), b.particles_get()):
+        assert list(p.position) == list(q.position) and list(p.torque) == list(q.torque)
+    for m in (False, True):
+        s = _particle_channel(orc, abi.F32, coupling, m)
+        s.step(3)
+        F64s, _, ABS = s.psm_couple()
+        for k, p in enumerate(s.particles_get()):
+            want = np.float32(F64s[k, :2])
+            got = np.array(p.force[:2], dtype=np.float32)
+            if m:
+                assert np.array_equal(got, want)
+            else:
+                assert np.all(np.abs(got - want) <= 1e-5 * ABS[k])
+
+
 # -------------------------------------------------------------- end to end
 def _run_channel(orc, nx, ny, steps, u_max=0.05, tau=0.8, st=F64):
     s = orc.Sim(abi.D2Q9, st, (nx, ny))
@@ -925,3 +975,33 @@ def test_fplbm_zero_sub_steps_does_not_move(orc):
     s.step(3)
     q = s.particles_get()[0]
     assert list(q.position)[:2] == list(p.position)[:2]
+
+
+def test_series_sine_of_the_porosity_ramp(orc):
+    """The engine's ramp sine (oracle restatement of the series the GPU
+    evaluates): within one ulp of the correctly rounded sin on [0, pi/2]
+    (glibc's double sin, through math.sin), and equal to it after rounding to
+    float; a disc stamped with it is within one ulp of the library sine's."""
+    xs = np.linspace(0.0, math.pi / 2 + 1e-3, 120001)
+    worst = 0.0
+    for x in xs:
+        a, b = orc.series_sin(float(x)), math.sin(float(x))
+        worst = max(worst, abs(a - b) / math.ulp(b) if b else abs(a))
+        assert np.float32(a) == np.float32(b)
+    assert worst <= 1.0
+    # the switch on a Sim: the stamped porosity with the series sine within one
+    # ulp of the library sine's
+    for st in (F64, abi.F32):
+        ps = []
+        for on in (False, True):
+            s = orc.Sim(abi.D2Q9, st, (40, 20))
+            s.set_flags(np.full(800, 2, np.uint8))
+            s.set_operator(2, abi.op_desc(collision=abi.COLL_PSM, tau=0.8, flags=abi.OPF_PSM_STAMP_DISC,
+                                          disc=(19.3, 9.7, 0.0, 6.2, 1.5)))
+            s.init_equilibrium(1.0, (0.0, 0.0))
+            s.set_series_sin(on)
+            s.step(1)
+            ps.append(s.get_porosity())
+        d = np.abs(ps[0] - ps[1])
+        assert np.sum((ps[0] > 0) & (ps[0] < 1)) > 20
+        assert np.all(d <= (4.5e-16 if st == F64 else 1.2e-7))
