@@ -292,10 +292,12 @@ lbm_status lbm_last_step_ms(lbm_ctx* ctx, double* ms);
  * The fused stream+collide kernel runs one row segment of block_x cells per
  * block, and addresses the population planes either with per-lane global
  * addresses (form 1) or through one buffer resource with the plane as a
- * scalar offset (form 2, where the array's bytes fit its 32-bit range);
+ * scalar offset (form 2, where the array's bytes fit its 32-bit range); with
+ * 16-bit storage and an even nx, form 3 gives each thread two cells whose
+ * values move as 32-bit words (block_x threads then cover 2*block_x cells);
  * form 0 takes the lattice's default.  Results are bitwise the same in every
  * shape and form.  lbm_autotune times every width (64, 128, 256, no wider
- * than the row) in both forms per AA phase on the context's own lattice over
+ * than the row) in every form that applies, per AA phase, on the context's own lattice over
  * `rounds` passes (a pass runs four even/odd step pairs back to back per
  * candidate, each launch between HIP events on the context's stream; the
  * mean per phase of a pass, the least over passes, is kept), restores
@@ -304,7 +306,7 @@ lbm_status lbm_last_step_ms(lbm_ctx* ctx, double* ms);
  * Call it from an even step count, after flags and operators are set. */
 typedef struct lbm_step_tuning {
   int32_t block_x;  /* threads per block along x: 64, 128 or 256     */
-  int32_t form[2];  /* per AA phase (even, odd): 0 default, 1 global, 2 buffer */
+  int32_t form[2];  /* per AA phase (even, odd): 0 default, 1 global, 2 buffer, 3 pairs */
   double us[2];     /* lbm_autotune: measured time per launch of each phase */
 } lbm_step_tuning;
 lbm_status lbm_autotune(lbm_ctx* ctx, uint32_t rounds, lbm_step_tuning* chosen);

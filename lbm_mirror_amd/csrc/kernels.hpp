@@ -43,7 +43,8 @@ struct LaunchShape {
   bool vec = true;       // vectorised kernel where the row length allows it
   // population addressing per AA phase: 0 the lattice's default, 1 global
   // addresses, 2 one buffer resource (used only where the array's bytes fit
-  // its 32-bit range)
+  // its 32-bit range), 3 two cells per thread as 32-bit words (16-bit
+  // storage, even rows)
   int32_t form[2] = {0, 0};
 };
 

@@ -945,8 +945,11 @@ lbm_status lbm_set_step_tuning(lbm_ctx* c, const lbm_step_tuning* t) {
   if (!c || !t) return fail(LBM_ERR_INVALID_ARG, "null argument");
   if (t->block_x != 64 && t->block_x != 128 && t->block_x != 256)
     return fail(LBM_ERR_INVALID_ARG, "block_x must be 64, 128 or 256");
-  for (int p = 0; p < 2; ++p)
-    if (t->form[p] < 0 || t->form[p] > 2) return fail(LBM_ERR_INVALID_ARG, "form must be 0, 1 or 2");
+  for (int p = 0; p < 2; ++p) {
+    if (t->form[p] < 0 || t->form[p] > 3) return fail(LBM_ERR_INVALID_ARG, "form must be 0, 1, 2 or 3");
+    if (t->form[p] == 3 && (storage_bytes(c->storage) != 2 || c->n[0] % 2))
+      return fail(LBM_ERR_NOT_SUPPORTED, "form 3 (two cells per thread) needs 16-bit storage and an even nx");
+  }
   c->shape.bx = t->block_x;
   c->shape.by = 1;
   c->shape.form[0] = t->form[0];
@@ -1018,11 +1021,13 @@ lbm_status lbm_autotune(lbm_ctx* c, uint32_t rounds, lbm_step_tuning* chosen) {
     if (uint64_t(bx) >= c->n[0]) break;
   }
   const int nw = int(widths.size());
-  std::vector<double> best_us(size_t(nw) * 2 * 2, 1e30);  // [width][form-1][phase]
+  // form 3 (two cells per thread) where the storage is 16-bit and rows even
+  const int nf = (storage_bytes(c->storage) == 2 && c->n[0] % 2 == 0) ? 3 : 2;
+  std::vector<double> best_us(size_t(nw) * 3 * 2, 1e30);  // [width][form-1][phase]
   const LaunchShape keep = c->shape;
   for (uint32_t r = 0; r < rounds; ++r)
     for (int wi = 0; wi < nw; ++wi)
-      for (int form = 1; form <= 2; ++form) {
+      for (int form = 1; form <= nf; ++form) {
         LaunchShape sh = keep;
         sh.bx = widths[wi];
         sh.by = 1;
@@ -1042,7 +1047,7 @@ lbm_status lbm_autotune(lbm_ctx* c, uint32_t rounds, lbm_step_tuning* chosen) {
           TUNE_TRY(hipEventElapsedTime(&m, ev[k], ev[k + 1]));
           sum[k & 1] += 1e3 * double(m);
         }
-        double* b = &best_us[(size_t(wi) * 2 + (form - 1)) * 2];
+        double* b = &best_us[(size_t(wi) * 3 + (form - 1)) * 2];
         b[0] = std::min(b[0], sum[0] / kPairs);
         b[1] = std::min(b[1], sum[1] / kPairs);
       }
@@ -1059,9 +1064,15 @@ lbm_status lbm_autotune(lbm_ctx* c, uint32_t rounds, lbm_step_tuning* chosen) {
     int f[2];
     double u[2];
     for (int p = 0; p < 2; ++p) {
-      const double g = best_us[(size_t(wi) * 2 + 0) * 2 + p], b = best_us[(size_t(wi) * 2 + 1) * 2 + p];
-      f[p] = b < g ? 2 : 1;
-      u[p] = std::min(g, b);
+      f[p] = 1;
+      u[p] = best_us[(size_t(wi) * 3 + 0) * 2 + p];
+      for (int form = 2; form <= nf; ++form) {
+        const double t = best_us[(size_t(wi) * 3 + (form - 1)) * 2 + p];
+        if (t < u[p]) {
+          u[p] = t;
+          f[p] = form;
+        }
+      }
     }
     if (u[0] + u[1] < bsum) {
       bsum = u[0] + u[1];

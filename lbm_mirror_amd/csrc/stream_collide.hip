@@ -593,6 +593,133 @@ __global__ __launch_bounds__(1024) void k_aa_row(StepArgs a) {
   if (x < a.nx) aa_row_cell<L, S, T, WB, ODD>(a, x, blockIdx.y, blockIdx.z);
 }
 
+// ---- 16-bit storage: two cells per thread ------------------------------------
+// A thread takes cells x0 = 2k and x0 + 1 of one row (one-row blocks) and
+// moves each plane's two 16-bit values as one 32-bit word: the FP32 kernel's
+// bytes per lane and per memory instruction at half the bytes per cell.  In
+// phase 1 the x-shifted words start at an odd element, an unaligned access
+// that gfx950's memory path serves directly (the compiler's unaligned-access
+// mode); where a pair would cross the row's end (periodic wrap, or the zero
+// ghost ring) its two values are moved one at a time.  Even rows only (nx
+// even); every element is still read and written by exactly one thread.
+template <class S>
+struct Bits16;
+template <>
+struct Bits16<__half> {
+  __device__ static float f(uint32_t b) { return __half2float(__ushort_as_half((unsigned short)b)); }
+  __device__ static uint32_t b(float v) { return __half_as_ushort(__float2half(v)); }
+};
+template <>
+struct Bits16<__hip_bfloat16> {
+  __device__ static float f(uint32_t b) { return __bfloat162float(__ushort_as_bfloat16((unsigned short)b)); }
+  __device__ static uint32_t b(float v) { return __bfloat16_as_ushort(__float2bfloat16(v)); }
+};
+
+#ifndef LBMX_NT_PAIR
+#define LBMX_NT_PAIR 3
+#endif
+template <class L, class S, bool ODD>
+__global__ __launch_bounds__(256) void k_aa_pair(StepArgs a) {
+  using T = float;
+  constexpr int Q = L::Q;
+  using G16 = __attribute__((address_space(1))) uint16_t;
+  using G32 = __attribute__((address_space(1))) uint32_t;
+  typedef uint32_t __attribute__((aligned(2))) u32_a2;  // a word at an even byte address
+  using G32u = __attribute__((address_space(1))) u32_a2;
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const int x0 = 2 * int(blockIdx.x * blockDim.x + threadIdx.x);
+  const int y = blockIdx.y, z = blockIdx.z;  // one-row blocks: uniform
+  if (x0 >= nx) return;                     // nx even: x0 + 1 < nx too
+  const uint32_t sxy = uint32_t(nx) * uint32_t(ny);
+  const uint32_t row = uint32_t(nx) * (uint32_t(y) + uint32_t(ny) * uint32_t(z));
+  G16* fb = (G16*)a.f;
+  const size_t Np = a.Np;
+  const bool px = a.periodic & 1u, py = a.periodic & 2u, pz = a.periodic & 4u;
+  const uint32_t Y[3] = {uint32_t(y == 0 ? ny - 1 : y - 1) * uint32_t(nx), uint32_t(y) * uint32_t(nx),
+                         uint32_t(y == ny - 1 ? 0 : y + 1) * uint32_t(nx)};
+  const uint32_t Z[3] = {uint32_t(z == 0 ? nz - 1 : z - 1) * sxy, uint32_t(z) * sxy,
+                         uint32_t(z == nz - 1 ? 0 : z + 1) * sxy};
+  const bool VY[3] = {y > 0 || py, true, y < ny - 1 || py};
+  const bool VZ[3] = {L::D < 3 || z > 0 || pz, true, L::D < 3 || z < nz - 1 || pz};
+  // the x position of x + d inside the row, or -1 in the ghost ring
+  auto wrap = [&](int x) -> int { return x < 0 ? (px ? x + nx : -1) : (x >= nx ? (px ? x - nx : -1) : x); };
+
+  T f[2][Q];
+  if (!ODD) {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const G32* p = (const G32*)(fb + (size_t(i) * Np + row + uint32_t(x0)));
+      const uint32_t v = (LBMX_NT_PAIR & 1) ? __builtin_nontemporal_load(p) : *p;
+      f[0][i] = Bits16<S>::f(v & 0xffffu) + T(L::w[i]);
+      f[1][i] = Bits16<S>::f(v >> 16) + T(L::w[i]);
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const int iy = 1 - L::cy[i], iz = 1 - L::cz[i];
+      const int xs = x0 - L::cx[i];  // the source of cell x0 (x0 + 1's is xs + 1)
+      const int p = L::opp[i];
+      if (!(VY[iy] && VZ[iz])) {
+        f[0][i] = f[1][i] = T(0);
+      } else if (xs >= 0 && xs + 1 < nx) {
+        const uint32_t v = *(const G32u*)(fb + (size_t(p) * Np + Y[iy] + Z[iz] + uint32_t(xs)));
+        f[0][i] = Bits16<S>::f(v & 0xffffu) + T(L::w[i]);
+        f[1][i] = Bits16<S>::f(v >> 16) + T(L::w[i]);
+      } else {
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+          const int xk = wrap(xs + k);
+          f[k][i] = xk >= 0 ? Bits16<S>::f(fb[size_t(p) * Np + Y[iy] + Z[iz] + uint32_t(xk)]) + T(L::w[i]) : T(0);
+        }
+      }
+    }
+  }
+
+  const uint32_t fl2 = ((const G16*)((const __attribute__((address_space(1))) uint8_t*)a.flags + row))[x0 >> 1];
+  collide_cell<L, T, false>(f[0], uint8_t(fl2 & 0xffu), a, row + uint32_t(x0), x0, y, z);
+  collide_cell<L, T, false>(f[1], uint8_t(fl2 >> 8), a, row + uint32_t(x0) + 1u, x0 + 1, y, z);
+
+  if (!ODD) {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const T w = T(L::w[i]);
+      const uint32_t v = Bits16<S>::b(f[0][i] - w) | (Bits16<S>::b(f[1][i] - w) << 16);
+      G32* p = (G32*)(fb + (size_t(L::opp[i]) * Np + row + uint32_t(x0)));
+      if (LBMX_NT_PAIR & 2) __builtin_nontemporal_store(v, p);
+      else *p = v;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const T w = T(L::w[i]);
+      const uint32_t b0 = Bits16<S>::b(f[0][i] - w), b1 = Bits16<S>::b(f[1][i] - w);
+      const int jy = 1 + L::cy[i], jz = 1 + L::cz[i];
+      const int xd = x0 + L::cx[i];  // the destination of cell x0 (x0 + 1's is xd + 1)
+      if (VY[jy] && VZ[jz]) {
+        G16* base = fb + (size_t(i) * Np + Y[jy] + Z[jz]);
+        if (xd >= 0 && xd + 1 < nx) {
+          *(G32u*)(base + uint32_t(xd)) = b0 | (b1 << 16);
+        } else {
+          const int d0 = wrap(xd), d1 = wrap(xd + 1);
+          if (d0 >= 0) base[d0] = (uint16_t)b0;
+          if (d1 >= 0) base[d1] = (uint16_t)b1;
+        }
+      }
+      // a cell whose upstream neighbour is in the ghost ring: its own slot
+      // takes the ghost's zero (nobody else streams into it)
+      const int uy = 1 - L::cy[i], uz = 1 - L::cz[i];
+      const bool rows_ok = VY[uy] && VZ[uz];
+      const bool up0 = rows_ok && wrap(x0 - L::cx[i]) >= 0, up1 = rows_ok && wrap(x0 + 1 - L::cx[i]) >= 0;
+      if (!up0 || !up1) {
+        G16* own = fb + (size_t(i) * Np + row + uint32_t(x0));
+        const uint32_t zb = Bits16<S>::b(T(0) - w);
+        if (!up0) own[0] = (uint16_t)zb;
+        if (!up1) own[1] = (uint16_t)zb;
+      }
+    }
+  }
+}
+
 // ---- vectorised variant ------------------------------------------------------
 // Each thread owns V consecutive cells of one row and moves every plane with
 // one aligned V-wide access.  In phase 1 the ±1 shifts along x are resolved
@@ -846,6 +973,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L::Q == 9 ?
     StepArgs a = b.members[tl.member];
     a.t = t;
     a.t_dev = nullptr;
+    // batch members report divergence through their residual (NaN); without
+    // the fault-word store the kernel keeps its registers (it spilled with it)
+    a.status = nullptr;
     a.write_macros = b.write_macros != 0;
     if (b.write_macros == 2) a.u = b.u_prev[tl.member];
 
@@ -952,6 +1082,24 @@ hipError_t launch_batch_residual(int dims, int storage, const BatchArgs& b, cons
 
 template <class L, class S, class T, bool WB>
 static hipError_t launch_t(bool odd, const StepArgs& a, const LaunchShape& sh, hipStream_t s) {
+  // 16-bit storage, even rows: two cells per thread (form 3; the default for
+  // D2Q9 and D3Q19, D3Q27's 54 values per thread spill); one-row blocks of
+  // sh.bx threads over the row's pairs
+  if constexpr (sizeof(S) == 2) {
+    const int form = sh.form[odd ? 1 : 0];
+    if ((form == 3 || (form == 0 && L::Q <= 19)) && a.nx % 2 == 0) {
+      const int pairs = a.nx / 2;
+      int bx = 64;
+      while (bx < pairs && bx < sh.bx) bx *= 2;
+      dim3 block(bx, 1, 1);
+      dim3 grid((pairs + bx - 1) / bx, a.ny, a.nz);
+      if (odd)
+        hipLaunchKernelGGL((k_aa_pair<L, S, true>), grid, block, 0, s, a);
+      else
+        hipLaunchKernelGGL((k_aa_pair<L, S, false>), grid, block, 0, s, a);
+      return hipGetLastError();
+    }
+  }
   if (a.nx >= kRowKernelMinNx) {
     // whole rows per block (up to 1024 cells), rows along grid y and z
     int bx = 64;

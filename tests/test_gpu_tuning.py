@@ -105,3 +105,36 @@ def test_autotune_needs_even_phase_and_set_up(hip):
     assert g.autotune(rounds=1)["block_x"] == 64
     with pytest.raises(LbmError):
         g.set_step_tuning(96)
+
+
+@pytest.mark.parametrize("lat,ext", [(abi.D3Q19, (260, 5, 4)), (abi.D2Q9, (300, 7)), (abi.D3Q27, (130, 4, 3)),
+                                     (abi.D3Q19, (64, 5, 4))])
+@pytest.mark.parametrize("st", [abi.F16S, abi.BF16S])
+def test_16bit_forms_are_bitwise_equal(hip, lat, st, ext):
+    """16-bit shifted storage: the scalar kernel in both addressing forms and
+    the two-cells-per-thread kernel (form 3) give the same bits, periodic and
+    not, over both AA phases and random operator mixtures."""
+    from lbm_mirror_amd.lattice import Lattice
+    for periodic in (0, 1):
+        results = []
+        for w, form in [(256, (1, 1)), (64, (2, 2)), (64, (3, 3)), (128, (3, 3)), (256, (3, 1)), (256, (1, 3))]:
+            g = _build([lambda l, s, e: Lattice(l, s, e)], lat, st, ext, 13)[0]
+            if not periodic:
+                g.set_periodic(0)
+            g.set_step_tuning(w, form)
+            g.step(5)
+            results.append(_state(g))
+            g.close()
+        for k, r in enumerate(results[1:], 1):
+            for a, b in zip(r, results[0]):
+                assert np.array_equal(a, b), f"case {k} periodic {periodic}"
+
+
+def test_pair_form_needs_16bit_and_even_rows(hip):
+    from lbm_mirror_amd import LbmError
+    from lbm_mirror_amd.lattice import Lattice
+    with pytest.raises(LbmError):
+        Lattice(abi.D2Q9, abi.F32, (64, 8)).set_step_tuning(64, (3, 3))
+    with pytest.raises(LbmError):
+        Lattice(abi.D2Q9, abi.F16S, (63, 8)).set_step_tuning(64, (3, 3))
+    Lattice(abi.D2Q9, abi.F16S, (64, 8)).set_step_tuning(64, (3, 3))
