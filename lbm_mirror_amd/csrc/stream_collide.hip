@@ -485,10 +485,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(StepWaves<L
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
       const int ix = 1 - L::cx[i], iy = 1 - L::cy[i], iz = 1 - L::cz[i];  // x − c_i
-      if (VX[ix] && VY[iy] && VZ[iz])
+      const bool inside = VX[ix] && VY[iy] && VZ[iz];
+      if constexpr (sizeof(S) == 2) {
+        // a 16-bit value is converted after its load: a load in a per-lane
+        // branch whose converted result leaves the branch holds the
+        // wavefront there until it returns (one memory latency per
+        // direction), so the load is unconditional (the wrapped neighbour
+        // always lies inside the lattice) and the ghost's zero is selected
+        // after it
+        const T v = ld(L::opp[i], Y[iy] + Z[iz], X[ix], T(L::w[i]));
+        f[i] = inside ? v : T(0);
+      } else if (inside) {
         f[i] = ld(L::opp[i], Y[iy] + Z[iz], X[ix], T(L::w[i]));
-      else
+      } else {
         f[i] = T(0);
+      }
     }
   }
 
@@ -561,8 +572,15 @@ __device__ __forceinline__ void aa_row_cell(const StepArgs& a, int x, int y, int
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
       const int ix = 1 - L::cx[i], iy = 1 - L::cy[i], iz = 1 - L::cz[i];  // x − c_i
-      f[i] = (VX[ix] && VY[iy] && VZ[iz]) ? Store<S, T>::ld(rowp(L::opp[i], iy, iz) + X[ix], T(L::w[i]))
-                                          : T(0);
+      const bool inside = VX[ix] && VY[iy] && VZ[iz];
+      if constexpr (sizeof(S) == 2) {
+        // unconditional (wrapped, in-lattice) load, then the ghost's zero:
+        // no 16-bit conversion inside a per-lane branch
+        const T v = Store<S, T>::ld(rowp(L::opp[i], iy, iz) + X[ix], T(L::w[i]));
+        f[i] = inside ? v : T(0);
+      } else {
+        f[i] = inside ? Store<S, T>::ld(rowp(L::opp[i], iy, iz) + X[ix], T(L::w[i])) : T(0);
+      }
     }
   }
 
@@ -599,9 +617,12 @@ __global__ __launch_bounds__(1024) void k_aa_row(StepArgs a) {
 // bytes per lane and per memory instruction at half the bytes per cell.  In
 // phase 1 the x-shifted words start at an odd element, an unaligned access
 // that gfx950's memory path serves directly (the compiler's unaligned-access
-// mode); where a pair would cross the row's end (periodic wrap, or the zero
-// ghost ring) its two values are moved one at a time.  Even rows only (nx
-// even); every element is still read and written by exactly one thread.
+// mode).  Phase-1 loads are all issued before any value is used (no load in
+// a per-lane branch); where a pair would cross the row's end (periodic wrap,
+// or the zero ghost ring) the word is clamped into the row and the missing
+// value loaded on its own, and such pairs are stored one value at a time.
+// Even rows only (nx even); every element is still read and written by
+// exactly one thread.
 template <class S>
 struct Bits16;
 template <>
@@ -654,24 +675,53 @@ __global__ __launch_bounds__(256) void k_aa_pair(StepArgs a) {
       f[1][i] = Bits16<S>::f(v >> 16) + T(L::w[i]);
     }
   } else {
+    // Every load is issued before any value is used: a load whose converted
+    // value left a per-lane branch would hold the wavefront there until it
+    // returned.  Direction i's sources are the word at x0 − c_x (cells x0 and
+    // x0 + 1), clamped into the row; a source row in the ghost ring reads this
+    // cell's own row instead (the value is then replaced by the ghost's zero).
+    // The row's first pair (c_x = +1) and last pair (c_x = −1) take one of
+    // their two values from the row's other end (periodic wrap) or the ghost.
+    uint32_t raw[Q];
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
       const int iy = 1 - L::cy[i], iz = 1 - L::cz[i];
-      const int xs = x0 - L::cx[i];  // the source of cell x0 (x0 + 1's is xs + 1)
-      const int p = L::opp[i];
-      if (!(VY[iy] && VZ[iz])) {
-        f[0][i] = f[1][i] = T(0);
-      } else if (xs >= 0 && xs + 1 < nx) {
-        const uint32_t v = *(const G32u*)(fb + (size_t(p) * Np + Y[iy] + Z[iz] + uint32_t(xs)));
-        f[0][i] = Bits16<S>::f(v & 0xffffu) + T(L::w[i]);
-        f[1][i] = Bits16<S>::f(v >> 16) + T(L::w[i]);
-      } else {
+      const uint32_t rb = (VY[iy] && VZ[iz]) ? Y[iy] + Z[iz] : row;
+      const int xs = x0 - L::cx[i];
+      const int xa = L::cx[i] > 0 ? (xs < 0 ? 0 : xs) : (L::cx[i] < 0 ? (xs > nx - 2 ? nx - 2 : xs) : xs);
+      raw[i] = *(const G32u*)(fb + (size_t(L::opp[i]) * Np + rb + uint32_t(xa)));
+    }
+    const bool first = x0 == 0, last = x0 == nx - 2;
+    uint32_t edge[Q];
 #pragma unroll
-        for (int k = 0; k < 2; ++k) {
-          const int xk = wrap(xs + k);
-          f[k][i] = xk >= 0 ? Bits16<S>::f(fb[size_t(p) * Np + Y[iy] + Z[iz] + uint32_t(xk)]) + T(L::w[i]) : T(0);
-        }
+    for (int i = 0; i < Q; ++i) edge[i] = 0u;
+    if (__any(first || last)) {  // the wavefronts that hold a row end
+#pragma unroll
+      for (int i = 0; i < Q; ++i) {
+        if (L::cx[i] == 0) continue;
+        const int iy = 1 - L::cy[i], iz = 1 - L::cz[i];
+        const uint32_t rb = (VY[iy] && VZ[iz]) ? Y[iy] + Z[iz] : row;
+        edge[i] = fb[size_t(L::opp[i]) * Np + rb + uint32_t(L::cx[i] > 0 ? nx - 1 : 0)];
       }
+    }
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const int iy = 1 - L::cy[i], iz = 1 - L::cz[i];
+      const bool rv = VY[iy] && VZ[iz];
+      uint32_t lo = raw[i] & 0xffffu, hi = raw[i] >> 16;
+      bool g0 = !rv, g1 = !rv;
+      if (L::cx[i] > 0) {
+        hi = first ? lo : hi;
+        lo = first ? edge[i] : lo;
+        g0 = g0 || (first && !px);
+      } else if (L::cx[i] < 0) {
+        lo = last ? hi : lo;
+        hi = last ? edge[i] : hi;
+        g1 = g1 || (last && !px);
+      }
+      const T w = T(L::w[i]);
+      f[0][i] = g0 ? T(0) : Bits16<S>::f(lo) + w;
+      f[1][i] = g1 ? T(0) : Bits16<S>::f(hi) + w;
     }
   }
 
