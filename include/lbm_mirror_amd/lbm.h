@@ -490,6 +490,14 @@ lbm_status lbm_net_train(lbm_net* net, uint32_t steps, float lr, uint64_t seed, 
  * solid cells rho 1, u 0). */
 lbm_status lbm_net_predict(lbm_net* net, uint32_t count, const uint8_t* flags, const double* u_max,
                            const double* tau, double* rho, double* u);
+/* Mass-flux correction of predicted velocities u[count*2N] (in place): in
+ * steady flow every column x carries the inlet's flux, u_max times the sum
+ * of the inlet parabola; each column whose predicted flux sum_y u_x exceeds
+ * a fifth of it is rescaled (u_x and u_y) to carry exactly that flux.  The
+ * network's column-wise flux errors are the slowest-decaying part of a warm
+ * start's error (no reference counterpart). */
+lbm_status lbm_net_correct_flux(lbm_net* net, uint32_t count, const uint8_t* flags, const double* u_max,
+                                double* u);
 /* Raw forward: x[count*N*8] features -> y[count*N*3] normalised outputs. */
 lbm_status lbm_net_forward_raw(lbm_net* net, uint32_t count, const float* x, float* y);
 /* Raw loss and gradient for the current parameters: features x, targets

@@ -39,7 +39,7 @@ EXPORTS = [
     "lbm_batch_run", "lbm_batch_autotune", "lbm_batch_set_tuning", "lbm_batch_stats",
     "lbm_init_equilibrium_field", "lbm_net_create", "lbm_net_destroy", "lbm_net_param_count",
     "lbm_net_get_params", "lbm_net_set_params", "lbm_net_features", "lbm_net_add_sample",
-    "lbm_net_clear_samples", "lbm_net_sample_count", "lbm_net_train", "lbm_net_predict",
+    "lbm_net_clear_samples", "lbm_net_sample_count", "lbm_net_train", "lbm_net_predict", "lbm_net_correct_flux",
     "lbm_net_forward_raw", "lbm_net_loss_grad_raw", "lbm_net_last_ms",
     "lbm_net_get_output_scale", "lbm_net_set_output_scale",
     "lbm_get_flags", "lbm_ctx_info", "lbm_write_vtk", "lbm_write_csv", "lbm_write_forst",
@@ -117,6 +117,7 @@ def load_library(path=None):
     L.lbm_net_sample_count.argtypes = [vp, C.POINTER(C.c_uint32)]
     L.lbm_net_train.argtypes = [vp, C.c_uint32, C.c_float, C.c_uint64, fp]
     L.lbm_net_predict.argtypes = [vp, C.c_uint32, u8p, dp, dp, dp, dp]
+    L.lbm_net_correct_flux.argtypes = [vp, C.c_uint32, u8p, dp, dp]
     L.lbm_net_forward_raw.argtypes = [vp, C.c_uint32, fp, fp]
     L.lbm_net_loss_grad_raw.argtypes = [vp, C.c_uint32, fp, fp, fp, dp, fp]
     L.lbm_net_last_ms.argtypes = [vp, dp]

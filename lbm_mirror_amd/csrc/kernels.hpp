@@ -152,6 +152,12 @@ struct ParticleArgs {
   int32_t fixed_axis;  // ParticleAxisIntegrate instead of ParticleIntegrate
   double axis[3];      // its normalised axis, T-rounded (2D: the sign in [0])
   uint32_t* status;    // the context's fault words (StepArgs::status)
+  // PSM scatter porosity without the per-cell reset: n*12 ints, particle p's
+  // box of this gather at [12p, 12p+6) (lo, hi) and of the previous one at
+  // [12p+6, 12p+12).  The gather sweeps the union of the two and sets the
+  // cells of the previous box that no current box holds back to 1 (null:
+  // every cell is reset before the gather)
+  int32_t* boxes;
 };
 
 // Up to this many particles the gather decides cell ownership per block (no
@@ -166,6 +172,9 @@ enum PartField {
 };
 
 hipError_t launch_psm_couple(int lattice, int storage, const ParticleArgs& a, hipStream_t s);
+// Porosity 1 in every cell and every particle's previous box empty: the
+// state from which the gathers keep the porosity without a per-cell reset
+hipError_t launch_sweep_reset(bool dbl, void* por, uint32_t N, int32_t* boxes, uint32_t n, hipStream_t s);
 // HLBM stamp after the move: N += u_p (1 - B), D += 1 - B over each post-move AABB
 hipError_t launch_hlbm_stamp(int dims, int storage, const ParticleArgs& a, hipStream_t s);
 hipError_t launch_dem_contact(int dims, int storage, const ParticleArgs& a, hipStream_t s);

@@ -131,6 +131,11 @@ struct lbm_ctx {
   uint64_t* cl_slot = nullptr;
   uint64_t* cl_overflow = nullptr;
   uint32_t* owner = nullptr;
+  // PSM scatter porosity kept by the gathers' sweeps (ParticleArgs::boxes):
+  // 12 ints per particle; por_fresh once the porosity is 1 outside the
+  // previous boxes (after a full reset, kept by every sweep since)
+  int32_t* boxes = nullptr;
+  bool por_fresh = false;
   double bin = 1.0;
   uint64_t grid[3] = {1, 1, 1};
   uint64_t bins = 1;
@@ -249,6 +254,23 @@ void spin_axis_normalised(int D, const double* in, double* out) {
 #endif
 constexpr uint64_t kGatherCells = LBMX_GATHER_CELLS;
 
+// PSM coupling with scatter porosity and per-block ownership: the gathers
+// keep the porosity without a per-cell reset
+bool sweep_eligible(const lbm_ctx* c) {
+  return c->np > 0 && c->boxes && c->pp.coupling == LBM_COUPLING_PSM &&
+         c->pp.porosity_mode == LBM_POROSITY_SCATTER && !c->owner;
+}
+bool sweep_ready(const lbm_ctx* c) { return sweep_eligible(c) && c->por_fresh; }
+
+// Before the couplings of a call: one full reset (porosity 1, no previous
+// boxes) where the porosity was set from outside or the particles changed
+lbm_status prepare_sweep(lbm_ctx* c) {
+  if (!sweep_eligible(c) || c->por_fresh) return LBM_OK;
+  HIP_TRY(launch_sweep_reset(c->storage == LBM_F64, c->por, c->N, c->boxes, c->np, c->stream));
+  c->por_fresh = true;
+  return LBM_OK;
+}
+
 ParticleArgs particle_args(lbm_ctx* c) {
   ParticleArgs a;
   std::memset(&a, 0, sizeof a);  // padding too: the bytes key the step graphs
@@ -267,6 +289,7 @@ ParticleArgs particle_args(lbm_ctx* c) {
   a.parts = c->parts;
   a.snap = c->snap;
   a.n = c->np;
+  a.boxes = sweep_ready(c) ? c->boxes : nullptr;
   a.f64 = c->f64;
   a.t64 = c->t64;
   a.partial = c->partial;
@@ -522,7 +545,7 @@ lbm_status lbm_ctx_destroy(lbm_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   void* ptrs[] = {c->f, c->flags, c->ops, c->rho, c->u, c->force, c->por, c->bcvel, c->pn, c->pd,
                   c->parts, c->snap, c->f64, c->t64, c->partial, c->cl_count, c->cl_slot,
-                  c->cl_overflow, c->owner, c->t_dev, c->u_prev, c->res_partial, c->res_out};
+                  c->cl_overflow, c->owner, c->t_dev, c->u_prev, c->res_partial, c->res_out, c->boxes};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->status_h) (void)hipHostFree(c->status_h);
@@ -668,6 +691,7 @@ lbm_status lbm_set_porosity(lbm_ctx* c, const double* p) {
   if (!c || !p) return fail(LBM_ERR_INVALID_ARG, "null argument");
   lbm_status s = ensure_field(c, &c->por, c->N, 1.0);
   if (s) return s;
+  c->por_fresh = false;  // the next coupling resets every cell, as PsmReset does
   return put_field(c, c->por, p, c->N);
 }
 
@@ -836,6 +860,8 @@ lbm_status lbm_step(lbm_ctx* c, uint64_t t0, uint64_t nsteps) {
   if (s) return s;
   HIP_TRY(hipSetDevice(c->device));
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
+  s = prepare_sweep(c);  // (outside any step graph)
+  if (s) return s;
   // FPLBM reads the stored macro velocity of non-fluid cells every step
   const bool every = c->np > 0 && c->pp.coupling == LBM_COUPLING_FPLBM;
   // residual tracking: the step before the last keeps its velocity in u_prev
@@ -1153,13 +1179,15 @@ lbm_status lbm_particles_set(lbm_ctx* c, const lbm_particle* p, uint32_t n,
   const bool dbl = compute_double(st);
   // (re)allocate
   if (n != c->np || dbl != c->pdbl) {
-    void* ptrs[] = {c->parts, c->snap, c->f64, c->t64, c->partial};
+    void* ptrs[] = {c->parts, c->snap, c->f64, c->t64, c->partial, c->boxes};
     for (void* q : ptrs)
       if (q) HIP_TRY(hipFree(q));
     c->parts = c->snap = nullptr;
     c->f64 = c->t64 = c->partial = nullptr;
+    c->boxes = nullptr;
     c->np = 0;
     if (n > 0) {
+      HIP_TRY(alloc_zero((void**)&c->boxes, size_t(n) * 12 * sizeof(int32_t), c->stream));
       HIP_TRY(alloc_zero(&c->parts, size_t(n) * kPartFields * (dbl ? 8 : 4), c->stream));
       HIP_TRY(alloc_zero(&c->snap, size_t(n) * kPartFields * (dbl ? 8 : 4), c->stream));
       HIP_TRY(alloc_zero((void**)&c->f64, size_t(n) * 3 * sizeof(double), c->stream));
@@ -1167,6 +1195,7 @@ lbm_status lbm_particles_set(lbm_ctx* c, const lbm_particle* p, uint32_t n,
     }
   }
   c->pp = *prm;
+  c->por_fresh = false;  // the next coupling starts from a full reset
   if (c->pp.cell_capacity <= 0) c->pp.cell_capacity = 4;
   // blocks per particle so that one block sweeps kGatherCells AABB cells: a
   // single large sphere (r 19, 69k cells) then spreads over 256 blocks
@@ -1282,6 +1311,8 @@ lbm_status lbm_psm_couple(lbm_ctx* c, uint64_t t) {
   if (!c) return fail(LBM_ERR_INVALID_ARG, "null ctx");
   if (c->np == 0) return fail(LBM_ERR_INVALID_STATE, "no particles");
   HIP_TRY(hipSetDevice(c->device));
+  lbm_status s = prepare_sweep(c);
+  if (s) return s;
   ParticleArgs a = particle_args(c);
   HIP_TRY(launch_psm_couple(c->lattice, c->storage, a, c->stream));
   return LBM_OK;

@@ -1152,6 +1152,34 @@ lbm_status lbm_net_predict(lbm_net* n, uint32_t count, const uint8_t* flags, con
   return LBM_OK;
 }
 
+lbm_status lbm_net_correct_flux(lbm_net* n, uint32_t count, const uint8_t* flags, const double* u_max, double* u) {
+  if (!n || !flags || !u_max || !u) return nfail(LBM_ERR_INVALID_ARG, "null argument");
+  const int H = n->cfg.height, W = n->cfg.width;
+  const size_t HW = size_t(H) * W;
+  double S = 0.0;  // the inlet parabola's flux per unit u_max (the features' profile)
+  for (int y = 1; y < H - 1; ++y) S += 4.0 * (y - 0.5) * (H - 1.5 - y) / ((H - 2.0) * (H - 2.0));
+  std::vector<double> q(W);
+  for (uint32_t k = 0; k < count; ++k) {
+    const uint8_t* fl = flags + k * HW;
+    double* ux = u + k * 2 * HW;
+    double* uy = ux + HW;
+    const double q_in = u_max[k] * S;
+    std::fill(q.begin(), q.end(), 0.0);
+    for (int y = 0; y < H; ++y)
+      for (int x = 0; x < W; ++x)
+        if (fl[size_t(y) * W + x] >= 2) q[x] += ux[size_t(y) * W + x];
+    for (int y = 0; y < H; ++y)
+      for (int x = 0; x < W; ++x) {
+        const size_t i = size_t(y) * W + x;
+        if (fl[i] < 2 || !(q[x] > 0.2 * q_in)) continue;
+        const double s = q_in / q[x];
+        ux[i] *= s;
+        uy[i] *= s;
+      }
+  }
+  return LBM_OK;
+}
+
 lbm_status lbm_net_loss_grad_raw(lbm_net* n, uint32_t count, const float* x, const float* t, const float* m,
                                  double* loss, float* grad) {
   if (!n || !x || !t || !m || !loss || !grad) return nfail(LBM_ERR_INVALID_ARG, "null argument");

@@ -354,11 +354,13 @@ struct LocalOwners {
   uint32_t* q;     // LDS: particle indices
   int (*box)[6];   // LDS: their boxes (lo, hi)
 
-  // every thread of the block calls this once
-  __device__ void build(const ParticleArgs& a, uint32_t p, const Box<T, D>& b) const {
+  // every thread of the block calls this once; all: every other particle
+  // whose box overlaps b (the sweep), else only those of higher index
+  __device__ void build(const ParticleArgs& a, uint32_t p, const Box<T, D>& b, bool all = false) const {
     if (threadIdx.x == 0) *cnt = 0;
     __syncthreads();
-    for (uint32_t j = p + 1 + threadIdx.x; j < a.n; j += blockDim.x) {
+    for (uint32_t j = (all ? 0u : p + 1) + threadIdx.x; j < a.n; j += blockDim.x) {
+      if (j == p) continue;
       const Box<T, D> o = aabb_of<T, D>(static_cast<const T*>(a.parts) + size_t(j) * kPartFields, a);
       bool hit = o.count > 0;
       for (int d = 0; d < D; ++d) hit = hit && o.lo[d] < b.hi[d] && b.lo[d] < o.hi[d];
@@ -384,6 +386,7 @@ struct LocalOwners {
     const int n = *cnt;
     if (n <= kOverlapCap) {
       for (int e = 0; e < n; ++e) {
+        if (q[e] < p) continue;  // (listed by a sweep: ownership goes to higher indices)
         bool in = true;
         for (int d = 0; d < D; ++d) in = in && x[d] >= box[e][d] && x[d] < box[e][3 + d];
         if (!in) continue;
@@ -407,6 +410,24 @@ struct LocalOwners {
       }
       if (!own_solid) return;
     }
+  }
+
+  // whether another particle's box holds the cell (after build(..., true)
+  // over a region that contains it)
+  __device__ bool held(const ParticleArgs& a, uint32_t p, const int* x) const {
+    const int n = *cnt;
+    if (n <= kOverlapCap) {
+      for (int e = 0; e < n; ++e) {
+        bool in = true;
+        for (int d = 0; d < D; ++d) in = in && x[d] >= box[e][d] && x[d] < box[e][3 + d];
+        if (in) return true;
+      }
+      return false;
+    }
+    for (uint32_t j = 0; j < a.n; ++j)
+      if (j != p && box_has<T, D>(aabb_of<T, D>(static_cast<const T*>(a.parts) + size_t(j) * kPartFields, a), x))
+        return true;
+    return false;
   }
 };
 
@@ -440,12 +461,51 @@ __global__ __launch_bounds__(256) void k_gather(ParticleArgs a) {
   __shared__ uint32_t lo_q[kOverlapCap];
   __shared__ int lo_box[kOverlapCap][6];
   const LocalOwners<T, D> lo{&lo_cnt, lo_q, lo_box};
-  if constexpr (LOCAL) lo.build(a, p, b);
+  // the sweep (PSM scatter porosity without the per-cell reset): the cells of
+  // this gather's box and of the previous gather's box, whose cells outside
+  // every current box go back to porosity 1
+  constexpr bool kCanSweep = CPL == kCplPsm && LOCAL;
+  const bool sweep = kCanSweep && a.boxes != nullptr && scatter;
+  Box<T, D> bp, R = b;
+  if (kCanSweep && sweep) {
+    const int32_t* pb = a.boxes + size_t(p) * 12 + 6;
+    bp.count = 1;
+    for (int d = 0; d < 3; ++d) {
+      bp.lo[d] = pb[d];
+      bp.hi[d] = pb[3 + d];
+      bp.count *= uint32_t(bp.hi[d] - bp.lo[d]);
+    }
+    if (b.count == 0) {
+      R = bp;
+    } else if (bp.count > 0) {
+      R.count = 1;
+      for (int d = 0; d < 3; ++d) {
+        R.lo[d] = min(b.lo[d], bp.lo[d]);
+        R.hi[d] = max(b.hi[d], bp.hi[d]);
+        R.count *= uint32_t(R.hi[d] - R.lo[d]);
+      }
+    }
+    // (constant indices only: a box indexed per lane would be placed in LDS,
+    // one copy per thread, halving the gather's occupancy)
+    if (chunk == 0 && threadIdx.x == 0) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        a.boxes[size_t(p) * 12 + d] = b.lo[d];
+        a.boxes[size_t(p) * 12 + 3 + d] = b.hi[d];
+      }
+    }
+  }
+  if constexpr (LOCAL) lo.build(a, p, R, sweep);
   double acc[6] = {0, 0, 0, 0, 0, 0};
-  for (uint32_t k = chunk * blockDim.x + threadIdx.x; k < b.count;
+  for (uint32_t k = chunk * blockDim.x + threadIdx.x; k < R.count;
        k += a.blocks_per_particle * blockDim.x) {
     int x[3];
-    box_cell<T, D>(b, k, x);
+    box_cell<T, D>(R, k, x);
+    if (kCanSweep && sweep && !box_has<T, D>(b, x)) {
+      if (box_has<T, D>(bp, x) && !lo.held(a, p, x))
+        por[uint32_t(x[0]) + uint32_t(a.nx) * (uint32_t(x[1]) + uint32_t(a.ny) * uint32_t(x[2]))] = F(1);
+      continue;
+    }
     T rel[3] = {T(0), T(0), T(0)};
     const T B = porosity_at<T, D>(P, x, eps, rel);
     const uint32_t c = uint32_t(x[0]) + uint32_t(a.nx) * (uint32_t(x[1]) + uint32_t(a.ny) * uint32_t(x[2]));
@@ -854,6 +914,9 @@ __global__ __launch_bounds__(64) void k_contact(ParticleArgs a) {
       }
     }
   }
+  // the gather's box of this step becomes the previous box of the next
+  if (a.boxes && a.fuse_finalize)
+    for (int k = 0; k < 6; ++k) a.boxes[size_t(s) * 12 + 6 + k] = a.boxes[size_t(s) * 12 + k];
   // only this particle's record is written; other blocks read the snapshot
   // (fused) or only positions, rotations, masses and radii (not fused)
   for (int k = 0; k < D; ++k) me[PF_FORCE + k] = Fs[k];
@@ -908,16 +971,33 @@ __global__ void k_integrate(ParticleArgs a) {
   integrate_one<T, D>(a, static_cast<T*>(a.parts) + size_t(s) * kPartFields);
 }
 
+// the gather's boxes become the previous boxes (couplings run on their own,
+// outside the full step's fused contact)
+__global__ void k_commit_boxes(int32_t* boxes, uint32_t n) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  for (int k = 0; k < 6; ++k) boxes[size_t(p) * 12 + 6 + k] = boxes[size_t(p) * 12 + k];
+}
+
+// no previous boxes: every particle's lo = hi = 0
+__global__ void k_clear_prev_boxes(int32_t* boxes, uint32_t n) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  for (int k = 0; k < 6; ++k) boxes[size_t(p) * 12 + 6 + k] = 0;
+}
+
 // ---------------------------------------------------------------- launchers
 template <class L, class S, class T, class F>
 static hipError_t couple_t(const ParticleArgs& a, hipStream_t s) {
   const uint32_t N = a.N;
   const bool scatter = a.porosity_mode == LBM_POROSITY_SCATTER;
   // one pass over the cells: PsmReset's porosity, the owners, HlbmReset's N
-  // and D, FpLbmReset's force
-  hipLaunchKernelGGL((k_reset_cells<F>), dim3((N + 255) / 256), dim3(256), 0, s, (F*)a.por, a.owner,
-                     a.hlbm ? (F*)a.pn : nullptr, a.hlbm ? (F*)a.pd : nullptr, a.fplbm ? (F*)a.force : nullptr,
-                     N, L::D, scatter ? 1 : 0);
+  // and D, FpLbmReset's force (the PSM sweep keeps the porosity without it)
+  const bool sweep = a.boxes != nullptr && scatter && !a.owner && !a.hlbm && !a.fplbm;
+  if (!sweep)
+    hipLaunchKernelGGL((k_reset_cells<F>), dim3((N + 255) / 256), dim3(256), 0, s, (F*)a.por, a.owner,
+                       a.hlbm ? (F*)a.pn : nullptr, a.hlbm ? (F*)a.pd : nullptr,
+                       a.fplbm ? (F*)a.force : nullptr, N, L::D, scatter ? 1 : 0);
   const dim3 pg(a.n * a.blocks_per_particle);
   if (a.owner) hipLaunchKernelGGL((k_owner<L, T>), pg, dim3(256), 0, s, a);
   const int cpl = a.hlbm ? kCplHlbm : (a.fplbm ? kCplFplbm : kCplPsm);
@@ -930,8 +1010,10 @@ static hipError_t couple_t(const ParticleArgs& a, hipStream_t s) {
     else if (cpl == kCplFplbm) hipLaunchKernelGGL((k_gather<L, S, T, F, kCplFplbm, true>), pg, dim3(256), 0, s, a);
     else hipLaunchKernelGGL((k_gather<L, S, T, F, kCplPsm, true>), pg, dim3(256), 0, s, a);
   }
-  if (!a.fuse_finalize)
+  if (!a.fuse_finalize) {
     hipLaunchKernelGGL((k_finalize<T, L::D>), dim3(a.n), dim3(64), 0, s, a);
+    if (sweep) hipLaunchKernelGGL(k_commit_boxes, dim3((a.n + 255) / 256), dim3(256), 0, s, a.boxes, a.n);
+  }
   if (!scatter) {
     uint64_t bins = a.grid[0] * a.grid[1] * a.grid[2];
     hipLaunchKernelGGL((k_build_cells<T, L::D>), dim3(1), dim3(1024), 0, s, a, bins);
@@ -958,6 +1040,17 @@ static hipError_t couple_l(int storage, const ParticleArgs& a, hipStream_t s) {
     case LBM_BF16S: return couple_t<L, __hip_bfloat16, float, float>(a, s);
   }
   return hipErrorInvalidValue;
+}
+
+hipError_t launch_sweep_reset(bool dbl, void* por, uint32_t N, int32_t* boxes, uint32_t n, hipStream_t s) {
+  if (dbl)
+    hipLaunchKernelGGL((k_reset_cells<double>), dim3((N + 255) / 256), dim3(256), 0, s, (double*)por, nullptr,
+                       nullptr, nullptr, nullptr, N, 0, 1);
+  else
+    hipLaunchKernelGGL((k_reset_cells<float>), dim3((N + 255) / 256), dim3(256), 0, s, (float*)por, nullptr,
+                       nullptr, nullptr, nullptr, N, 0, 1);
+  if (n > 0) hipLaunchKernelGGL(k_clear_prev_boxes, dim3((n + 255) / 256), dim3(256), 0, s, boxes, n);
+  return hipGetLastError();
 }
 
 hipError_t launch_psm_couple(int lattice, int storage, const ParticleArgs& a, hipStream_t s) {

@@ -101,6 +101,16 @@ class Net:
         check(self.L.lbm_net_predict(self._h, k, _u8(f), _dp(um), _dp(ta), _dp(rho), _dp(u)))
         return rho.reshape(k, N), u.reshape(k, 2, N)
 
+    def correct_flux(self, flags_list, u_max, u):
+        """Rescale each column of predicted velocities u (count, 2, N) to the
+        inlet's flux (lbm_net_correct_flux); returns the corrected copy."""
+        k = len(flags_list)
+        f = np.ascontiguousarray(np.stack([np.asarray(x, dtype=np.uint8).reshape(-1) for x in flags_list]))
+        um = np.ascontiguousarray(u_max, dtype=np.float64)
+        out = np.ascontiguousarray(np.array(u, dtype=np.float64).reshape(k, 2, -1))
+        check(self.L.lbm_net_correct_flux(self._h, k, _u8(f), _dp(um), _dp(out)))
+        return out
+
     def forward_raw(self, x):
         x = np.ascontiguousarray(x, dtype=np.float32)
         k = x.size // (self.height * self.width * self.FEATURES)
