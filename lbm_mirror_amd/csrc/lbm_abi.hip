@@ -431,7 +431,10 @@ lbm_status replay_steps(lbm_ctx* c, uint64_t t, bool every) {
     append_bytes(key, pa);
   }
   if (!c->step_graph || key != c->graph_key) {
-    if (c->step_graph) HIP_TRY(hipGraphExecDestroy(c->step_graph));
+    if (c->step_graph) {  // (after its queued replays)
+      HIP_TRY(hipStreamSynchronize(c->stream));
+      HIP_TRY(hipGraphExecDestroy(c->step_graph));
+    }
     c->step_graph = nullptr;
     hipGraph_t g = nullptr;
     HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -1412,7 +1415,12 @@ StepArgs member_args(const lbm_ctx* c) {
 }
 
 void drop_chain(lbm_batch* b) {
-  if (b->chain) (void)hipGraphExecDestroy(b->chain);
+  // a launch of the chain may still be queued: it must finish before the
+  // executable graph goes
+  if (b->chain) {
+    (void)hipStreamSynchronize(b->stream);
+    (void)hipGraphExecDestroy(b->chain);
+  }
   b->chain = nullptr;
   b->chain_len = 0;
   b->chain_res = -1;
