@@ -426,6 +426,107 @@ def bench_c5(args):
                        "final_loss": float(losses[-50:].mean()), "wall_s": round(time.perf_counter() - t_all, 1)},
             "roofline": None}
 
+def bench_c5_study(args):
+    """What a warm start must achieve on the config-5 members (no network):
+    held-out members run to residual 1e-6 from rest in FP64 and in FP32
+    (the FP32 rounding floor), then the members that converged in FP64 run
+    again from the equilibrium of their own converged field with the
+    errors a prediction could have: none, rho = 1, u scaled by 1.03 and
+    1.10, and smooth random velocity errors (wavenumbers <= 8) of 3% and 10%
+    relative L2.  value = cold / warm steps (sum) from the exact field."""
+    import numpy as np
+    from helpers import c4_member
+    from lbm_mirror_amd import abi
+    side, tol = 256, 1e-6
+    n = args.test_samples
+    t_all = time.perf_counter()
+    rng = np.random.default_rng(20260107)
+    mem = [c4_member(rng, side_lo=side, side_hi=side) for _ in range(n)]
+
+    def to_steady(b, label, count):
+        total = np.zeros(count)
+        res = np.full(count, np.inf)
+        done = np.zeros(count, bool)
+        t, ran = 0, 0
+        while not done.all() and ran < args.test_max_steps:
+            seg = min(25000, args.test_max_steps - ran)
+            st, r = b.run(tol=tol, check_every=100, max_steps=seg, t0=t)
+            act = ~done
+            total[act] += st[act]
+            res[act] = r[act]
+            done |= (r < tol) | np.isnan(r)
+            t += int(st.max())
+            ran += seg
+            print(f"# [{time.perf_counter() - t_all:.0f} s] {label}: {ran} steps, "
+                  f"{int(np.sum(res < tol))} converged", file=sys.stderr, flush=True)
+        return total, res
+
+    out = {"members": n}
+    b32 = _c5_batch(mem, storage=abi.F32)
+    _, r32 = to_steady(b32, "cold FP32", n)
+    b32.close()
+    fin = np.isfinite(r32)
+    out["fp32"] = {"converged": int(np.sum(r32 < tol)), "nan": int(np.sum(~fin)),
+                   "stalled_1e-6_to_1e-5": int(np.sum(fin & (r32 >= tol) & (r32 < 1e-5)))}
+    b64 = _c5_batch(mem)
+    sc, r64 = to_steady(b64, "cold FP64", n)
+    ok = np.flatnonzero(r64 < tol)
+    fields = [b64.members[i].get_macros() for i in ok]
+    b64.close()
+    fin = np.isfinite(r64)
+    out["fp64"] = {"converged": int(ok.size), "nan": int(np.sum(~fin)),
+                   "stalled_1e-6_to_1e-5": int(np.sum(fin & (r64 >= tol) & (r64 < 1e-5))),
+                   "median_steps": float(np.median(sc[ok])) if ok.size else None}
+    sub = [mem[i] for i in ok]
+    sc_ok = sc[ok]
+    gen = np.random.default_rng(5)
+
+    def smooth(shape, kmax):
+        F = np.fft.rfft2(gen.standard_normal(shape))
+        ky = np.abs(np.fft.fftfreq(shape[0]) * shape[0])[:, None]
+        kx = (np.fft.rfftfreq(shape[1]) * shape[1])[None, :]
+        F[(ky > kmax) | (kx > kmax)] = 0
+        return np.fft.irfft2(F, s=shape)
+
+    def noisy(u, fl, eps):
+        nse = np.stack([smooth(fl.shape, 8), smooth(fl.shape, 8)]) * fl
+        return (u + eps * np.linalg.norm(u[:, fl]) / np.linalg.norm(nse[:, fl]) * nse) * fl
+
+    variants = {"exact": lambda r, u, fl: (r, u),
+                "exact_u_rho1": lambda r, u, fl: (np.ones_like(r), u),
+                "u_x1.03_rho1": lambda r, u, fl: (np.ones_like(r), 1.03 * u),
+                "u_x1.10_rho1": lambda r, u, fl: (np.ones_like(r), 1.10 * u),
+                "u_noise0.03_rho1": lambda r, u, fl: (np.ones_like(r), noisy(u, fl, 0.03)),
+                "u_noise0.10_rho1": lambda r, u, fl: (np.ones_like(r), noisy(u, fl, 0.10))}
+    out["warm"] = {}
+    for name, fn in variants.items():
+        rr, uu = [], []
+        for k, i in enumerate(ok):
+            rho, u = fields[k]
+            fl = mem[i][1].reshape(side, side) >= 2
+            r0, u0 = fn(rho.reshape(side, side), u.reshape(2, side, side), fl)
+            rr.append(r0.reshape(-1))
+            uu.append(u0.reshape(-1))
+        b = _c5_batch(sub, warm=(rr, uu))
+        sw, rw = to_steady(b, name, len(sub))
+        b.close()
+        both = rw < tol
+        out["warm"][name] = {"converged": int(both.sum()),
+                             "step_ratio_sum": round(float(sc_ok[both].sum() / max(sw[both].sum(), 1)), 3)
+                             if both.any() else None,
+                             "step_ratio_median": round(float(np.median(sc_ok[both] / np.maximum(sw[both], 1))), 3)
+                             if both.any() else None}
+    exact = out["warm"]["exact"]["step_ratio_sum"]
+    return {"metric": "time-to-steady (residual 1e-6) cold / warm start from the exact converged field",
+            "value": exact, "unit": "x", "n_gpus": 1, "steps": None, "warmup": 0, "ms_per_step": None,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64 (and f32 cold)",
+            "data": "synthetic",
+            "config": dict(out, workload=f"D2Q9 256x256 config-4 members, {n} seeds, residual 1e-6, "
+                                         f"at most {args.test_max_steps} steps",
+                           wall_s=round(time.perf_counter() - t_all, 1)),
+            "roofline": None}
+
+
 def bench_writers(args):
     """Result output on the headline lattice (D3Q19 FP32 512x128x128, the
     channel after 20 steps): legacy VTK and CSV of the drivers' MacroStruct
@@ -605,10 +706,12 @@ def main():
                          "(times the power-up transient with the default shape)")
     ap.add_argument("--tune-rounds", type=int, default=12,
                     help="passes of lbm_autotune over the launch shapes before the warm-up")
-    ap.add_argument("--config", default="c2", choices=["c2", "c2b", "c1", "c3", "c4", "c5", "writers", "sphere3d"],
+    ap.add_argument("--config", default="c2", choices=["c2", "c2b", "c1", "c3", "c4", "c5", "c5study", "writers",
+                                                     "sphere3d"],
                     help="c2: the headline D3Q19 channel (default); c2b: D3Q27 256^3; "
                          "c1: D2Q9 FP64 channel; c3: particle-laden; c4: ragged batch; "
-                         "c5: learned warm start; writers: VTK/CSV/Forst output; "
+                         "c5: learned warm start; c5study: the warm start's required accuracy "
+                         "(no network); writers: VTK/CSV/Forst output; "
                          "sphere3d: the 3D HLBM sphere / rotor drivers")
     ap.add_argument("--train-samples", type=int, default=10000)
     ap.add_argument("--test-samples", type=int, default=1000)
@@ -638,6 +741,9 @@ def main():
         args.lattice = "D3Q27"
     if args.config == "c5":
         print(json.dumps(bench_c5(args)), flush=True)
+        return
+    if args.config == "c5study":
+        print(json.dumps(bench_c5_study(args)), flush=True)
         return
     if args.config == "writers":
         print(json.dumps(bench_writers(args)), flush=True)
