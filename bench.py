@@ -527,6 +527,46 @@ def bench_c5_study(args):
             "roofline": None}
 
 
+def bench_nettrain(args):
+    """Training step of the 256x256 warm-start network (batch 16, bf16 MFMA
+    kernels) on synthetic samples (a straight channel, random fields):
+    device time per Adam step over --steps steps after --warmup, and the
+    dense-bf16 fraction of the convolutions' flops (forward, data gradient
+    and weight gradient of the 12 3x3 layers)."""
+    import numpy as np
+    from helpers import channel_flags_2d
+    from lbm_mirror_amd.warmstart import Net
+    H = W = 256
+    widths = (32, 64, 96, 160)
+    B = 16
+    net = Net(H, W, widths, batch=B, capacity=32, seed=1)
+    rng = np.random.default_rng(0)
+    for _ in range(32):
+        u = rng.standard_normal((2, H * W)) * 0.01
+        rho = 1 + rng.standard_normal(H * W) * 1e-4
+        net.add_sample(channel_flags_2d(W, H), 0.05, 0.8, rho, u)
+    w = widths
+    spec = [(8, w[0], 0), (w[0], w[0], 0), (w[0], w[1], 1), (w[1], w[1], 1), (w[1], w[2], 2), (w[2], w[2], 2),
+            (w[2], w[3], 3), (w[3], w[3], 3), (w[3] + w[2], w[2], 2), (w[2] + w[1], w[1], 1),
+            (w[1] + w[0], w[0], 0), (w[0], 3, 0)]
+    flops = sum(3 * 2 * 9 * ci * co * B * (H >> l) * (W >> l) for ci, co, l in spec)
+    if args.warmup:
+        net.train(args.warmup, 1e-3, 0)
+    steps = args.steps if args.steps != 1000 else 100
+    net.train(steps, 1e-3, 1)
+    ms = net.last_ms() / steps
+    tflops = flops / (ms * 1e-3) / 1e12
+    return {"metric": "training step of the warm-start network (256x256, batch 16)", "value": round(ms, 4),
+            "unit": "ms", "n_gpus": 1, "steps": steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+            "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "bf16 (fp32 accumulate)",
+            "data": "synthetic",
+            "config": {"workload": "encoder-decoder 32/64/96/160, 12 3x3 convolutions, 916,451 parameters, "
+                                   "Adam, batch 16 of 256x256x8 features",
+                       "gflop_per_step": round(flops / 1e9, 1)},
+            "roofline": {"bound": "mfma", "achieved": round(tflops, 1), "peak": 2500.0, "unit": "TFLOP/s",
+                         "frac": round(tflops / 2500.0, 4), "traffic": None}}
+
+
 def bench_writers(args):
     """Result output on the headline lattice (D3Q19 FP32 512x128x128, the
     channel after 20 steps): legacy VTK and CSV of the drivers' MacroStruct
@@ -706,12 +746,12 @@ def main():
                          "(times the power-up transient with the default shape)")
     ap.add_argument("--tune-rounds", type=int, default=12,
                     help="passes of lbm_autotune over the launch shapes before the warm-up")
-    ap.add_argument("--config", default="c2", choices=["c2", "c2b", "c1", "c3", "c4", "c5", "c5study", "writers",
-                                                     "sphere3d"],
+    ap.add_argument("--config", default="c2", choices=["c2", "c2b", "c1", "c3", "c4", "c5", "c5study", "nettrain",
+                                                     "writers", "sphere3d"],
                     help="c2: the headline D3Q19 channel (default); c2b: D3Q27 256^3; "
                          "c1: D2Q9 FP64 channel; c3: particle-laden; c4: ragged batch; "
                          "c5: learned warm start; c5study: the warm start's required accuracy "
-                         "(no network); writers: VTK/CSV/Forst output; "
+                         "(no network); nettrain: the network's training step; writers: VTK/CSV/Forst output; "
                          "sphere3d: the 3D HLBM sphere / rotor drivers")
     ap.add_argument("--train-samples", type=int, default=10000)
     ap.add_argument("--test-samples", type=int, default=1000)
@@ -744,6 +784,9 @@ def main():
         return
     if args.config == "c5study":
         print(json.dumps(bench_c5_study(args)), flush=True)
+        return
+    if args.config == "nettrain":
+        print(json.dumps(bench_nettrain(args)), flush=True)
         return
     if args.config == "writers":
         print(json.dumps(bench_writers(args)), flush=True)

@@ -12,10 +12,12 @@
 // over pixel ranges with a fixed-order reduction of the partial tiles, so
 // training is deterministic run to run.
 //
-// Block tile 128 (M) x 32 (N) x 32 (K): four 64-wide waves, each owning a
-// 32x32 sub-tile = 2x2 MFMA 16x16 blocks.  A and B tiles are staged through
-// LDS with the K index contiguous so that every MFMA operand is one 16-byte
-// LDS read (rows padded by 8 elements against bank conflicts).
+// Block tile 128 (M) x 16*NC (N) x 32 (K), NC per layer so that the
+// layer's channel count is a whole number of tiles (up to 128 columns):
+// four 64-wide waves, each owning 32 rows x 16*NC columns = 2 x NC MFMA
+// 16x16 blocks.  A and B tiles are staged through two LDS buffers with the K
+// index contiguous, so that every MFMA operand is one 16-byte LDS read (rows
+// padded by 8 elements against bank conflicts); one barrier per K step.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -48,7 +50,7 @@ struct Act {  // element (pixel q, channel c) at p[q*ld + off + c]
   int ld, off;
 };
 
-constexpr int BM = 128, BN = 32, BK = 32, LK = BK + 8;
+constexpr int BM = 128, BK = 32, LK = BK + 8;
 
 struct ConvArgs {
   int B, H, W;
@@ -70,37 +72,40 @@ __device__ inline uint4 load_tap(const Act& s, int H, int W, int b, int y, int x
   return *reinterpret_cast<const uint4*>(s.p + q * s.ld + s.off + c);
 }
 
-__device__ inline void mma_tile(const uint16_t* As, const uint16_t* Bs, int wave, int lane, f32x4 (&acc)[2][2]) {
-  bf16x8 af[2], bfr[2];
+// A wavefront's 32 rows of the block tile against NC 16-column blocks of the
+// B tile (2 x NC MFMA 16x16x32 per K step of 32).
+template <int NC>
+__device__ inline void mma_tile_n(const uint16_t* As, const uint16_t* Bs, int wave, int lane,
+                                  f32x4 (&acc)[2][NC]) {
+  bf16x8 af[2];
 #pragma unroll
   for (int r = 0; r < 2; ++r)
     af[r] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(
                                            &As[(wave * 32 + r * 16 + (lane & 15)) * LK + (lane >> 4) * 8]));
 #pragma unroll
-  for (int c = 0; c < 2; ++c)
-    bfr[c] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(
-                                            &Bs[(c * 16 + (lane & 15)) * LK + (lane >> 4) * 8]));
+  for (int c = 0; c < NC; ++c) {
+    const bf16x8 bfr = __builtin_bit_cast(
+        bf16x8, *reinterpret_cast<const uint4*>(&Bs[(c * 16 + (lane & 15)) * LK + (lane >> 4) * 8]));
 #pragma unroll
-  for (int r = 0; r < 2; ++r)
-#pragma unroll
-    for (int c = 0; c < 2; ++c) acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[r], bfr[c], acc[r][c], 0, 0, 0);
-}
-
-__device__ inline void scatter8(uint16_t* base, int stride, uint4 v) {
-  const uint32_t w[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    base[(2 * t) * stride] = uint16_t(w[t] & 0xffffu);
-    base[(2 * t + 1) * stride] = uint16_t(w[t] >> 16);
+    for (int r = 0; r < 2; ++r) acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[r], bfr, acc[r][c], 0, 0, 0);
   }
 }
 
 // Forward convolution and data gradient (implicit GEMM over 3x3 taps).
+// Block tile BM = 128 pixels x 16*NC output channels (NC chosen per layer so
+// the channel count is a whole number of tiles), K in steps of 32 through
+// two LDS buffers: the next step's A (gathered taps) and B (weights, stored
+// N-major with K contiguous, so both tiles go to LDS without transposing)
+// are loaded into registers while the current step's MFMAs run, then written
+// to the other buffer; one barrier per K step.
+template <int NC>
 __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
-  __shared__ __align__(16) uint16_t As[BM * LK];
-  __shared__ __align__(16) uint16_t Bs[BN * LK];
+  constexpr int BNc = 16 * NC;
+  constexpr int NB = (BNc * 4 + 255) / 256;  // B chunks of 8 per thread
+  __shared__ __align__(16) uint16_t As[2][BM * LK];
+  __shared__ __align__(16) uint16_t Bs[2][BNc * LK];
   const int M = a.B * a.H * a.W, K = 9 * a.cin, N = a.cout, HW = a.H * a.W;
-  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BNc;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int kc = (tid & 3) * 8;
   int rb[2], ry[2], rx[2];
@@ -115,7 +120,7 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
     ry[j] = r / a.W;
     rx[j] = r - ry[j] * a.W;
   }
-  uint4 ra[2], rbv = make_uint4(0u, 0u, 0u, 0u);
+  uint4 ra[2], rw[NB];
   auto gload = [&](int k0) {
     const int kk = k0 + kc;
     int tap = 0, c = 0;
@@ -126,38 +131,43 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
 #pragma unroll
     for (int j = 0; j < 2; ++j)
       ra[j] = (rv[j] && kk < K) ? load_tap(a.in, a.H, a.W, rb[j], ry[j], rx[j], tap, c) : make_uint4(0u, 0u, 0u, 0u);
-    if (tid < 128) {
-      const int kb = k0 + (tid >> 2), n = n0 + (tid & 3) * 8;
-      rbv = (kb < K && n < N) ? *reinterpret_cast<const uint4*>(a.w + size_t(kb) * N + n) : make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+      const int i = tid + 256 * q;
+      const int n = n0 + (i >> 2), k = k0 + (i & 3) * 8;
+      rw[q] = (i < BNc * 4 && n < N && k < K) ? *reinterpret_cast<const uint4*>(a.w + size_t(n) * K + k)
+                                               : make_uint4(0u, 0u, 0u, 0u);
     }
   };
-  auto sstore = [&]() {
+  auto sstore = [&](int buf) {
 #pragma unroll
-    for (int j = 0; j < 2; ++j) *reinterpret_cast<uint4*>(&As[((tid + 256 * j) >> 2) * LK + kc]) = ra[j];
-    if (tid < 128) scatter8(&Bs[((tid & 3) * 8) * LK + (tid >> 2)], LK, rbv);
+    for (int j = 0; j < 2; ++j) *reinterpret_cast<uint4*>(&As[buf][((tid + 256 * j) >> 2) * LK + kc]) = ra[j];
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+      const int i = tid + 256 * q;
+      if (i < BNc * 4) *reinterpret_cast<uint4*>(&Bs[buf][(i >> 2) * LK + (i & 3) * 8]) = rw[q];
+    }
   };
-  f32x4 acc[2][2];
+  f32x4 acc[2][NC];
 #pragma unroll
   for (int r = 0; r < 2; ++r)
 #pragma unroll
-    for (int c = 0; c < 2; ++c) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int c = 0; c < NC; ++c) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
   const int nk = (K + BK - 1) / BK;
   gload(0);
-  sstore();
+  sstore(0);
   __syncthreads();
   for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
     if (kt + 1 < nk) gload((kt + 1) * BK);
-    mma_tile(As, Bs, wave, lane, acc);
+    mma_tile_n<NC>(As[cur], Bs[cur], wave, lane, acc);
+    if (kt + 1 < nk) sstore(cur ^ 1);  // read by no wave since the last barrier
     __syncthreads();
-    if (kt + 1 < nk) {
-      sstore();
-      __syncthreads();
-    }
   }
 #pragma unroll
   for (int r = 0; r < 2; ++r)
 #pragma unroll
-    for (int c = 0; c < 2; ++c)
+    for (int c = 0; c < NC; ++c)
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int m = m0 + wave * 32 + r * 16 + (lane >> 4) * 4 + j;
@@ -173,6 +183,27 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
           a.out.p[size_t(m) * a.out.ld + a.out.off + n] = f2bf(v);
         }
       }
+}
+
+// 16-column blocks per conv tile for N output channels: the widest tile of
+// at most 128 channels that divides N (N = 8, the output layer, takes 32)
+int conv_nc(int N) {
+  for (int nc : {8, 6, 5, 4, 2})
+    if (N % (16 * nc) == 0) return nc;
+  return 2;
+}
+
+hipError_t launch_conv(const ConvArgs& a, hipStream_t s) {
+  const int M = a.B * a.H * a.W, nc = conv_nc(a.cout);
+  const dim3 grid((M + BM - 1) / BM, (a.cout + 16 * nc - 1) / (16 * nc));
+  switch (nc) {
+    case 8: hipLaunchKernelGGL(k_conv<8>, grid, dim3(256), 0, s, a); break;
+    case 6: hipLaunchKernelGGL(k_conv<6>, grid, dim3(256), 0, s, a); break;
+    case 5: hipLaunchKernelGGL(k_conv<5>, grid, dim3(256), 0, s, a); break;
+    case 4: hipLaunchKernelGGL(k_conv<4>, grid, dim3(256), 0, s, a); break;
+    default: hipLaunchKernelGGL(k_conv<2>, grid, dim3(256), 0, s, a); break;
+  }
+  return hipGetLastError();
 }
 
 struct WgradArgs {
@@ -193,15 +224,21 @@ __device__ inline void scatter8x2(uint16_t* base, int stride, uint4 v0, uint4 v1
   }
 }
 
-// Weight gradient: per split of the pixel range, one partial 128x32 tile of
-// dW[(tap,ci)][co] = sum_p in[p+off(tap)][ci] * dpre[p][co].  Row 9*cin of the
-// im2col operand is 1 on every pixel, so the same GEMM yields the bias
-// gradient sum_p dpre[p][co] in that row (rows 9*cin+1 .. +7 are zero).
+// Weight gradient: per split of the pixel range, one partial 128 x 16*NC tile
+// of dW[(tap,ci)][co] = sum_p in[p+off(tap)][ci] * dpre[p][co].  Row 9*cin of
+// the im2col operand is 1 on every pixel, so the same GEMM yields the bias
+// gradient sum_p dpre[p][co] in that row (rows 9*cin+1 .. +7 are zero).  The
+// K dimension is pixels: both operands arrive channel-contiguous and are
+// transposed on their way into LDS, two pixels per 32-bit write; two LDS
+// buffers, the next step's tiles loaded while the current one's MFMAs run.
+template <int NC>
 __global__ __launch_bounds__(256) void k_wgrad(WgradArgs a) {
-  __shared__ __align__(16) uint16_t As[BM * LK];
-  __shared__ __align__(16) uint16_t Bs[BN * LK];
+  constexpr int BNc = 16 * NC;
+  constexpr int BCH = BNc / 8;  // 8-channel chunks of a B row block per pixel
+  __shared__ __align__(16) uint16_t As[2][BM * LK];
+  __shared__ __align__(16) uint16_t Bs[2][BNc * LK];
   const int M = a.B * a.H * a.W, Mw = 9 * a.cin, Mx = Mw + 8, N = a.cout, HW = a.H * a.W;
-  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN, s = blockIdx.z;
+  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BNc, s = blockIdx.z;
   const int p0 = s * a.chunk, p1 = min(p0 + a.chunk, M);
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int g = tid & 15;
@@ -209,8 +246,9 @@ __global__ __launch_bounds__(256) void k_wgrad(WgradArgs a) {
   const bool mv = m8 < Mw, ones = m8 == Mw;
   const int tap = mv ? m8 / a.cin : 0;
   const int ci = mv ? m8 - tap * a.cin : 0;
-  const int kq = 2 * (tid >> 4);              // A: pixel pair (kq, kq+1) of the K step
-  const int kb = 2 * ((tid & 63) >> 2), nc = (tid & 3) * 8;  // B: pixel pair, channel chunk (tid < 64)
+  const int kq = 2 * (tid >> 4);  // A: pixel pair (kq, kq+1) of the K step
+  const bool bt = tid < 16 * BCH;  // B: 16 pixel pairs x BCH chunks
+  const int kb = 2 * (tid / BCH), nc = (tid % BCH) * 8;
   uint4 ra[2], rb[2];
   auto gload = [&](int pb) {
 #pragma unroll
@@ -226,7 +264,7 @@ __global__ __launch_bounds__(256) void k_wgrad(WgradArgs a) {
         }
       }
     }
-    if (tid < 64) {
+    if (bt) {
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
         const int q = pb + kb + j, n = n0 + nc;
@@ -235,35 +273,33 @@ __global__ __launch_bounds__(256) void k_wgrad(WgradArgs a) {
       }
     }
   };
-  auto sstore = [&]() {
-    scatter8x2(&As[(g * 8) * LK + kq], LK, ra[0], ra[1]);
-    if (tid < 64) scatter8x2(&Bs[nc * LK + kb], LK, rb[0], rb[1]);
+  auto sstore = [&](int buf) {
+    scatter8x2(&As[buf][(g * 8) * LK + kq], LK, ra[0], ra[1]);
+    if (bt) scatter8x2(&Bs[buf][nc * LK + kb], LK, rb[0], rb[1]);
   };
-  f32x4 acc[2][2];
+  f32x4 acc[2][NC];
 #pragma unroll
   for (int r = 0; r < 2; ++r)
 #pragma unroll
-    for (int c = 0; c < 2; ++c) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int c = 0; c < NC; ++c) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
   const int nk = p1 > p0 ? (p1 - p0 + BK - 1) / BK : 0;
   if (nk > 0) {
     gload(p0);
-    sstore();
+    sstore(0);
     __syncthreads();
   }
   for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
     if (kt + 1 < nk) gload(p0 + (kt + 1) * BK);
-    mma_tile(As, Bs, wave, lane, acc);
+    mma_tile_n<NC>(As[cur], Bs[cur], wave, lane, acc);
+    if (kt + 1 < nk) sstore(cur ^ 1);  // read by no wave since the last barrier
     __syncthreads();
-    if (kt + 1 < nk) {
-      sstore();
-      __syncthreads();
-    }
   }
   float* out = a.partial + size_t(s) * Mx * N;
 #pragma unroll
   for (int r = 0; r < 2; ++r)
 #pragma unroll
-    for (int c = 0; c < 2; ++c)
+    for (int c = 0; c < NC; ++c)
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int m = m0 + wave * 32 + r * 16 + (lane >> 4) * 4 + j;
@@ -272,16 +308,37 @@ __global__ __launch_bounds__(256) void k_wgrad(WgradArgs a) {
       }
 }
 
+hipError_t launch_wgrad(const WgradArgs& a, int splits, hipStream_t s) {
+  const int nc = conv_nc(a.cout);
+  const dim3 grid((9 * a.cin + 8 + BM - 1) / BM, (a.cout + 16 * nc - 1) / (16 * nc), splits);
+  switch (nc) {
+    case 8: hipLaunchKernelGGL(k_wgrad<8>, grid, dim3(256), 0, s, a); break;
+    case 6: hipLaunchKernelGGL(k_wgrad<6>, grid, dim3(256), 0, s, a); break;
+    case 5: hipLaunchKernelGGL(k_wgrad<5>, grid, dim3(256), 0, s, a); break;
+    case 4: hipLaunchKernelGGL(k_wgrad<4>, grid, dim3(256), 0, s, a); break;
+    default: hipLaunchKernelGGL(k_wgrad<2>, grid, dim3(256), 0, s, a); break;
+  }
+  return hipGetLastError();
+}
+
 // grad[m*cr + n] = sum_s partial[s][m][n] for m <= 9*cin, n < cr (the
 // unpadded Cout); row 9*cin is the bias, which follows the weights in the
-// flat parameter layout.
-__global__ void k_wreduce(const float* partial, int splits, int Mw, int cpad, int cr, float* grad) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (Mw + 1) * cr) return;
-  const int m = i / cr, n = i - m * cr;
+// flat parameter layout.  A block takes 64 consecutive outputs: four groups
+// of 64 threads add the splits k = g, g+4, ... in order, then the four sums
+// are added in group order (a fixed order: training repeats bit for bit).
+__global__ __launch_bounds__(256) void k_wreduce(const float* partial, int splits, int Mw, int cpad, int cr,
+                                                 float* grad) {
+  __shared__ float part[4][64];
+  const int gq = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int i = blockIdx.x * 64 + l;
+  const bool ok = i < (Mw + 1) * cr;
+  const int m = ok ? i / cr : 0, n = ok ? i - m * cr : 0;
   float s = 0.f;
-  for (int k = 0; k < splits; ++k) s += partial[(size_t(k) * (Mw + 8) + m) * cpad + n];
-  grad[i] = s;
+  if (ok)
+    for (int k = gq; k < splits; k += 4) s += partial[(size_t(k) * (Mw + 8) + m) * cpad + n];
+  part[gq][l] = s;
+  __syncthreads();
+  if (gq == 0 && ok) grad[i] = ((part[0][l] + part[1][l]) + part[2][l]) + part[3][l];
 }
 
 // 2x2 average pool of C channels: fine (H x W) -> coarse (H/2 x W/2).
@@ -412,14 +469,16 @@ __global__ void k_adam(float* w, const float* g, float* m1, float* m2, size_t n,
   w[i] = w[i] - lr * (a / c1) / (sqrtf(b / c2) + eps);
 }
 
-// FP32 master [9][cin][cr] -> bf16 W [9][cin][cpad] and Wt [9][cpad][cin] (taps flipped).
+// FP32 master [9][cin][cr] -> the bf16 B operands of the conv GEMMs, N-major
+// with K contiguous: forward W[co][tap*cin + ci], data gradient
+// Wt[ci][(8 - tap)*cpad + co] (taps flipped).
 __global__ void k_wconvert(const float* wm, int cin, int cr, int cpad, uint16_t* wb, uint16_t* wt) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= 9 * cin * cpad) return;
   const int co = i % cpad, ci = (i / cpad) % cin, tap = i / (cpad * cin);
   const uint16_t v = co < cr ? f2bf(wm[(size_t(tap) * cin + ci) * cr + co]) : uint16_t(0);
-  wb[i] = v;
-  wt[(size_t(8 - tap) * cpad + co) * cin + ci] = v;
+  wb[size_t(co) * (9 * cin) + tap * cin + ci] = v;
+  wt[size_t(ci) * (9 * cpad) + (8 - tap) * cpad + co] = v;
 }
 
 // In-place standardisation of stored targets: t = (t - mu) / sd.
@@ -571,7 +630,8 @@ Act act(uint16_t* p, int ld, int off = 0) { return Act{p, ld, off}; }
 // Splits of the pixel range for a weight-gradient launch of layer L.
 void wgrad_shape(const lbm_net* n, const Layer& L, int B, int& splits, int& chunk) {
   const int M = B * H_(n, L.level) * W_(n, L.level);
-  const int tiles = ((9 * L.cin + 8 + BM - 1) / BM) * ((L.cpad + BN - 1) / BN);
+  const int bn = 16 * conv_nc(L.cpad);
+  const int tiles = ((9 * L.cin + 8 + BM - 1) / BM) * ((L.cpad + bn - 1) / bn);
   splits = std::max(1, std::min(1024 / std::max(1, tiles), (M + 255) / 256));
   chunk = ((M + splits - 1) / splits + BK - 1) / BK * BK;
   splits = (M + chunk - 1) / chunk;
@@ -594,10 +654,7 @@ lbm_status conv(lbm_net* n, int li, int B, Act in, Act out, float* outf) {
   a.outf_n = L.cr;
   a.relu = L.relu;
   a.mask = Act{nullptr, 0, 0};
-  const int M = B * a.H * a.W;
-  dim3 grid((M + BM - 1) / BM, (a.cout + BN - 1) / BN);
-  hipLaunchKernelGGL(k_conv, grid, dim3(256), 0, n->stream, a);
-  NET_TRY(hipGetLastError());
+  NET_TRY(launch_conv(a, n->stream));
   return LBM_OK;
 }
 
@@ -610,11 +667,9 @@ lbm_status conv_bwd(lbm_net* n, int li, int B, Act in, Act dpre, Act dx, bool ma
   int splits, chunk;
   wgrad_shape(n, L, B, splits, chunk);
   WgradArgs wa{B, H, W, L.cin, L.cpad, in, dpre, n->partial, chunk};
-  dim3 wg((9 * L.cin + 8 + BM - 1) / BM, (L.cpad + BN - 1) / BN, splits);
-  hipLaunchKernelGGL(k_wgrad, wg, dim3(256), 0, n->stream, wa);
-  NET_TRY(hipGetLastError());
+  NET_TRY(launch_wgrad(wa, splits, n->stream));
   const int Mw = 9 * L.cin;
-  hipLaunchKernelGGL(k_wreduce, dim3(((Mw + 1) * L.cr + 255) / 256), dim3(256), 0, n->stream, n->partial, splits, Mw,
+  hipLaunchKernelGGL(k_wreduce, dim3(((Mw + 1) * L.cr + 63) / 64), dim3(256), 0, n->stream, n->partial, splits, Mw,
                      L.cpad, L.cr, n->g + L.w_off);
   NET_TRY(hipGetLastError());
   if (dx.p) {
@@ -632,9 +687,7 @@ lbm_status conv_bwd(lbm_net* n, int li, int B, Act in, Act dpre, Act dx, bool ma
     a.outf = nullptr;
     a.relu = 0;
     a.mask = mask_in ? in : Act{nullptr, 0, 0};
-    dim3 grid((M + BM - 1) / BM, (L.cin + BN - 1) / BN);
-    hipLaunchKernelGGL(k_conv, grid, dim3(256), 0, n->stream, a);
-    NET_TRY(hipGetLastError());
+    NET_TRY(launch_conv(a, n->stream));
   }
   return LBM_OK;
 }
