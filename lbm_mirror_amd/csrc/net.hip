@@ -250,18 +250,34 @@ __global__ __launch_bounds__(256) void k_wgrad(WgradArgs a) {
   const bool bt = tid < 16 * BCH;  // B: 16 pixel pairs x BCH chunks
   const int kb = 2 * (tid / BCH), nc = (tid % BCH) * 8;
   uint4 ra[2], rb[2];
+  // (b, y, x) of this thread's first pixel kq of the current K step, advanced
+  // by BK pixels per step without divisions
+  int pb_ = p0 + kq;
+  int cb = pb_ / HW, cy = (pb_ - cb * HW) / a.W, cx = pb_ - cb * HW - cy * a.W;
   auto gload = [&](int pb) {
+    int b = cb, y = cy, x = cx;
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       const int q = pb + kq + j;
       ra[j] = make_uint4(0u, 0u, 0u, 0u);
       if (q < p1) {
-        if (mv) {
-          const int b = q / HW, r = q - b * HW, y = r / a.W, x = r - y * a.W;
-          ra[j] = load_tap(a.in, a.H, a.W, b, y, x, tap, ci);
-        } else if (ones) {
-          ra[j].x = 0x3f80u;  // bf16 1.0 in row 9*cin
+        if (mv) ra[j] = load_tap(a.in, a.H, a.W, b, y, x, tap, ci);
+        else if (ones) ra[j].x = 0x3f80u;  // bf16 1.0 in row 9*cin
+      }
+      if (++x == a.W) {  // the pair's second pixel
+        x = 0;
+        if (++y == a.H) {
+          y = 0;
+          ++b;
         }
+      }
+    }
+    cx += BK;
+    while (cx >= a.W) {
+      cx -= a.W;
+      if (++cy == a.H) {
+        cy = 0;
+        ++cb;
       }
     }
     if (bt) {
