@@ -241,14 +241,18 @@ __global__ __launch_bounds__(256) void k_wgrad(WgradArgs a) {
   const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BNc, s = blockIdx.z;
   const int p0 = s * a.chunk, p1 = min(p0 + a.chunk, M);
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const int g = tid & 15;
+  // The 16 lanes of a quarter-wavefront take 16 consecutive pixel pairs of
+  // one 8-row group, so their transposing 32-bit LDS writes fall in 16
+  // consecutive banks (row groups a multiple of 32 banks apart would
+  // otherwise collide 8 ways)
+  const int g = tid >> 4;
   const int m8 = m0 + g * 8;
   const bool mv = m8 < Mw, ones = m8 == Mw;
   const int tap = mv ? m8 / a.cin : 0;
   const int ci = mv ? m8 - tap * a.cin : 0;
-  const int kq = 2 * (tid >> 4);  // A: pixel pair (kq, kq+1) of the K step
+  const int kq = 2 * (tid & 15);  // A: pixel pair (kq, kq+1) of the K step
   const bool bt = tid < 16 * BCH;  // B: 16 pixel pairs x BCH chunks
-  const int kb = 2 * (tid / BCH), nc = (tid % BCH) * 8;
+  const int kb = 2 * (tid & 15), nc = (tid >> 4) * 8;
   uint4 ra[2], rb[2];
   // (b, y, x) of this thread's first pixel kq of the current K step, advanced
   // by BK pixels per step without divisions
