@@ -361,6 +361,24 @@ __global__ __launch_bounds__(256) void k_wreduce(const float* partial, int split
   if (gq == 0 && ok) grad[i] = ((part[0][l] + part[1][l]) + part[2][l]) + part[3][l];
 }
 
+// 8 channels (16 bytes, 16-byte aligned: channel offsets and row pitches are
+// multiples of 8) as one vector access
+__device__ inline void ld8(const uint16_t* p, float (&f)[8]) {
+  const uint4 v = *reinterpret_cast<const uint4*>(p);
+  const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    f[2 * t] = bf2f(uint16_t(w[t] & 0xffffu));
+    f[2 * t + 1] = bf2f(uint16_t(w[t] >> 16));
+  }
+}
+__device__ inline void st8(uint16_t* p, const float (&f)[8]) {
+  uint32_t w[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) w[t] = uint32_t(f2bf(f[2 * t])) | (uint32_t(f2bf(f[2 * t + 1])) << 16);
+  *reinterpret_cast<uint4*>(p) = make_uint4(w[0], w[1], w[2], w[3]);
+}
+
 // 2x2 average pool of C channels: fine (H x W) -> coarse (H/2 x W/2).
 __global__ void k_pool_fwd(Act src, Act dst, int B, int H, int W, int C) {
   const int C8 = C / 8, h = H / 2, w = W / 2;
@@ -374,9 +392,14 @@ __global__ void k_pool_fwd(Act src, Act dst, int B, int H, int W, int C) {
   const uint16_t* p01 = p00 + src.ld;
   const uint16_t* p10 = p00 + size_t(W) * src.ld;
   const uint16_t* p11 = p10 + src.ld;
-  uint16_t* o = dst.p + q * dst.ld + dst.off + c;
+  float a00[8], a01[8], a10[8], a11[8], o[8];
+  ld8(p00, a00);
+  ld8(p01, a01);
+  ld8(p10, a10);
+  ld8(p11, a11);
 #pragma unroll
-  for (int e = 0; e < 8; ++e) o[e] = f2bf((((bf2f(p00[e]) + bf2f(p01[e])) + bf2f(p10[e])) + bf2f(p11[e])) * 0.25f);
+  for (int e = 0; e < 8; ++e) o[e] = (((a00[e] + a01[e]) + a10[e]) + a11[e]) * 0.25f;
+  st8(dst.p + q * dst.ld + dst.off + c, o);
 }
 
 // Pool backward into the skip gradient: g[p] = g[p] + (act[p] > 0 ? dpool[parent]/4 : 0).
@@ -388,14 +411,17 @@ __global__ void k_pool_bwd(Act dpool, Act gskip, Act act, int B, int H, int W, i
   const size_t p = i / C8;
   const int x = int(p % W), y = int((p / W) % H), b = int(p / (size_t(W) * H));
   const size_t par = (size_t(b) * (H / 2) + y / 2) * (W / 2) + x / 2;
-  uint16_t* g = gskip.p + p * gskip.ld + gskip.off + c;
-  const uint16_t* v = act.p + p * act.ld + act.off + c;
-  const uint16_t* d = dpool.p + par * dpool.ld + dpool.off + c;
+  uint16_t* gp = gskip.p + p * gskip.ld + gskip.off + c;
+  float g[8], v[8], d[8];
+  ld8(gp, g);
+  ld8(act.p + p * act.ld + act.off + c, v);
+  ld8(dpool.p + par * dpool.ld + dpool.off + c, d);
 #pragma unroll
   for (int e = 0; e < 8; ++e) {
-    const float add = bf2f(v[e]) > 0.f ? bf2f(d[e]) * 0.25f : 0.f;
-    g[e] = f2bf(bf2f(g[e]) + add);
+    const float add = v[e] > 0.f ? d[e] * 0.25f : 0.f;
+    g[e] = g[e] + add;
   }
+  st8(gp, g);
 }
 
 // Nearest 2x upsample: dst fine pixel p = src coarse parent(p).
@@ -424,9 +450,14 @@ __global__ void k_up_bwd(Act gfine, Act gcoarse, int B, int H, int W, int C) {
   const uint16_t* p01 = p00 + gfine.ld;
   const uint16_t* p10 = p00 + size_t(W) * gfine.ld;
   const uint16_t* p11 = p10 + gfine.ld;
-  uint16_t* o = gcoarse.p + q * gcoarse.ld + gcoarse.off + c;
+  float a00[8], a01[8], a10[8], a11[8], o[8];
+  ld8(p00, a00);
+  ld8(p01, a01);
+  ld8(p10, a10);
+  ld8(p11, a11);
 #pragma unroll
-  for (int e = 0; e < 8; ++e) o[e] = f2bf(((bf2f(p00[e]) + bf2f(p01[e])) + bf2f(p10[e])) + bf2f(p11[e]));
+  for (int e = 0; e < 8; ++e) o[e] = ((a00[e] + a01[e]) + a10[e]) + a11[e];
+  st8(gcoarse.p + q * gcoarse.ld + gcoarse.off + c, o);
 }
 
 // Masked MSE over 3 output channels: gy[q][c] = bf16(2 (y - t) / (3 nf)) on
