@@ -102,8 +102,12 @@ template <int NC>
 __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
   constexpr int BNc = 16 * NC;
   constexpr int NB = (BNc * 4 + 255) / 256;  // B chunks of 8 per thread
-  __shared__ __align__(16) uint16_t As[2][BM * LK];
-  __shared__ __align__(16) uint16_t Bs[2][BNc * LK];
+  // the two tile buffers; after the K loop the same LDS holds the block's
+  // bf16 results for 16-byte stores (128 rows of BNc + 8 values)
+  constexpr int kTiles = 2 * BM * LK + 2 * BNc * LK, kOut = BM * (BNc + 8);
+  __shared__ __align__(16) uint16_t smem[kTiles > kOut ? kTiles : kOut];
+  uint16_t (*As)[BM * LK] = reinterpret_cast<uint16_t (*)[BM * LK]>(smem);
+  uint16_t (*Bs)[BNc * LK] = reinterpret_cast<uint16_t (*)[BNc * LK]>(smem + 2 * BM * LK);
   const int M = a.B * a.H * a.W, K = 9 * a.cin, N = a.cout, HW = a.H * a.W;
   const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BNc;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -164,25 +168,60 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
     if (kt + 1 < nk) sstore(cur ^ 1);  // read by no wave since the last barrier
     __syncthreads();
   }
+  if (a.outf) {  // FP32 result (the output layer)
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int m = m0 + wave * 32 + r * 16 + (lane >> 4) * 4 + j;
+          const int n = n0 + c * 16 + (lane & 15);
+          if (m >= M || n >= a.outf_n) continue;
+          float v = acc[r][c][j];
+          if (a.bias && n < a.bias_n) v = v + a.bias[n];
+          if (a.relu) v = v > 0.f ? v : 0.f;
+          a.outf[size_t(m) * a.outf_n + n] = v;
+        }
+    return;
+  }
+  // bias and ReLU into the LDS tile (the K loop ended with a barrier) ...
+  constexpr int OP = BNc + 8;
 #pragma unroll
   for (int r = 0; r < 2; ++r)
 #pragma unroll
     for (int c = 0; c < NC; ++c)
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const int m = m0 + wave * 32 + r * 16 + (lane >> 4) * 4 + j;
-        const int n = n0 + c * 16 + (lane & 15);
-        if (m >= M || n >= N) continue;
+        const int row = wave * 32 + r * 16 + (lane >> 4) * 4 + j, col = c * 16 + (lane & 15);
+        const int n = n0 + col;
         float v = acc[r][c][j];
         if (a.bias && n < a.bias_n) v = v + a.bias[n];
         if (a.relu) v = v > 0.f ? v : 0.f;
-        if (a.mask.p && !(bf2f(a.mask.p[size_t(m) * a.mask.ld + a.mask.off + n]) > 0.f)) v = 0.f;
-        if (a.outf) {
-          if (n < a.outf_n) a.outf[size_t(m) * a.outf_n + n] = v;
-        } else {
-          a.out.p[size_t(m) * a.out.ld + a.out.off + n] = f2bf(v);
-        }
+        smem[row * OP + col] = f2bf(v);
       }
+  __syncthreads();
+  // ... then 8 channels per 16-byte store, with the input-gradient mask
+  // (zero where the layer input was not positive) applied per element
+  constexpr int CH = BNc / 8;
+  for (int t = tid; t < BM * CH; t += 256) {
+    const int row = t / CH, ch = t - row * CH;
+    const int m = m0 + row, n = n0 + ch * 8;
+    if (m >= M || n >= N) continue;
+    uint4 v = *reinterpret_cast<const uint4*>(&smem[row * OP + ch * 8]);
+    if (a.mask.p) {
+      const uint4 mk = *reinterpret_cast<const uint4*>(a.mask.p + size_t(m) * a.mask.ld + a.mask.off + n);
+      uint32_t w[4] = {v.x, v.y, v.z, v.w};
+      const uint32_t mw[4] = {mk.x, mk.y, mk.z, mk.w};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (!(bf2f(uint16_t(mw[q] & 0xffffu)) > 0.f)) w[q] &= 0xffff0000u;
+        if (!(bf2f(uint16_t(mw[q] >> 16)) > 0.f)) w[q] &= 0x0000ffffu;
+      }
+      v = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+    *reinterpret_cast<uint4*>(a.out.p + size_t(m) * a.out.ld + a.out.off + n) = v;
+  }
 }
 
 // 16-column blocks per conv tile for N output channels: the widest tile of
