@@ -383,8 +383,8 @@ hipError_t launch_wgrad(const WgradArgs& a, int splits, hipStream_t s) {
 // grad[m*cr + n] = sum_s partial[s][m][n] for m <= 9*cin, n < cr (the
 // unpadded Cout); row 9*cin is the bias, which follows the weights in the
 // flat parameter layout.  A block takes 64 consecutive outputs: four groups
-// of 64 threads add the splits k = g, g+4, ... in order, then the four sums
-// are added in group order (a fixed order: training repeats bit for bit).
+// of 64 threads add the splits k = g, g+4, ..., then the four sums are added
+// in group order (a fixed order: training repeats bit for bit).
 __global__ __launch_bounds__(256) void k_wreduce(const float* partial, int splits, int Mw, int cpad, int cr,
                                                  float* grad) {
   __shared__ float part[4][64];
@@ -392,10 +392,20 @@ __global__ __launch_bounds__(256) void k_wreduce(const float* partial, int split
   const int i = blockIdx.x * 64 + l;
   const bool ok = i < (Mw + 1) * cr;
   const int m = ok ? i / cr : 0, n = ok ? i - m * cr : 0;
-  float s = 0.f;
-  if (ok)
-    for (int k = gq; k < splits; k += 4) s += partial[(size_t(k) * (Mw + 8) + m) * cpad + n];
-  part[gq][l] = s;
+  // four running sums per thread (splits gq + 4(4i + u), u = 0..3) keep
+  // four loads in flight; combined in a fixed order
+  float su[4] = {0.f, 0.f, 0.f, 0.f};
+  if (ok) {
+    const size_t stride = size_t(Mw + 8) * cpad;
+    const float* src = partial + size_t(m) * cpad + n;
+    int k = gq;
+    for (; k + 12 < splits; k += 16) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) su[u] += src[size_t(k + 4 * u) * stride];
+    }
+    for (int u = 0; k < splits; k += 4, ++u) su[u] += src[size_t(k) * stride];
+  }
+  part[gq][l] = (su[0] + su[1]) + (su[2] + su[3]);
   __syncthreads();
   if (gq == 0 && ok) grad[i] = ((part[0][l] + part[1][l]) + part[2][l]) + part[3][l];
 }
@@ -717,12 +727,17 @@ void build_layers(lbm_net* n) {
 
 Act act(uint16_t* p, int ld, int off = 0) { return Act{p, ld, off}; }
 
-// Splits of the pixel range for a weight-gradient launch of layer L.
+// Splits of the pixel range for a weight-gradient launch of layer L: about
+// kWgradBlocks blocks (tiles x splits), at least 256 pixels per split.
+#ifndef LBMNET_WGRAD_BLOCKS
+#define LBMNET_WGRAD_BLOCKS 1024
+#endif
+constexpr int kWgradBlocks = LBMNET_WGRAD_BLOCKS;
 void wgrad_shape(const lbm_net* n, const Layer& L, int B, int& splits, int& chunk) {
   const int M = B * H_(n, L.level) * W_(n, L.level);
   const int bn = 16 * conv_nc(L.cpad);
   const int tiles = ((9 * L.cin + 8 + BM - 1) / BM) * ((L.cpad + bn - 1) / bn);
-  splits = std::max(1, std::min(1024 / std::max(1, tiles), (M + 255) / 256));
+  splits = std::max(1, std::min(kWgradBlocks / std::max(1, tiles), (M + 255) / 256));
   chunk = ((M + splits - 1) / splits + BK - 1) / BK * BK;
   splits = (M + chunk - 1) / chunk;
 }
