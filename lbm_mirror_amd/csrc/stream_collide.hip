@@ -564,23 +564,18 @@ __device__ __forceinline__ void aa_row_cell(const StepArgs& a, int x, int y, int
   if (!ODD) {
 #pragma unroll
     for (int i = 0; i < Q; ++i) f[i] = ld_even<S, T>(rowp(i, 1, 1) + X[1], T(L::w[i]));
-  } else if (inner) {
-#pragma unroll
-    for (int i = 0; i < Q; ++i)
-      f[i] = Store<S, T>::ld(rowp(L::opp[i], 1 - L::cy[i], 1 - L::cz[i]) + X[1 - L::cx[i]], T(L::w[i]));
   } else {
+    // one unconditional (wrapped, in-lattice) load per direction, then the
+    // ghost's zero: no loads in per-lane branches.  (An all-inner arm beside a
+    // predicated arm made the compiler wait for every load in flight before
+    // the second arm reused the first arm's destination registers: one memory
+    // latency more per cell.)
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
       const int ix = 1 - L::cx[i], iy = 1 - L::cy[i], iz = 1 - L::cz[i];  // x − c_i
       const bool inside = VX[ix] && VY[iy] && VZ[iz];
-      if constexpr (sizeof(S) == 2) {
-        // unconditional (wrapped, in-lattice) load, then the ghost's zero:
-        // no 16-bit conversion inside a per-lane branch
-        const T v = Store<S, T>::ld(rowp(L::opp[i], iy, iz) + X[ix], T(L::w[i]));
-        f[i] = inside ? v : T(0);
-      } else {
-        f[i] = inside ? Store<S, T>::ld(rowp(L::opp[i], iy, iz) + X[ix], T(L::w[i])) : T(0);
-      }
+      const T v = Store<S, T>::ld(rowp(L::opp[i], iy, iz) + X[ix], T(L::w[i]));
+      f[i] = inside ? v : T(0);
     }
   }
 
