@@ -312,46 +312,64 @@ __device__ __forceinline__ void zouhe_p_d2q9(T* f, bool dir, T rs) {
 // tangential terms of the reconstruction.
 template <class T>
 __device__ __forceinline__ void zouhe_p_d3q27(T* f, bool east, T rs, bool normal_only = false) {
+  // Both faces' sums and reconstructions are formed and the face's values are
+  // selected: assignments to face-dependent elements would become stores
+  // through a selected address, which keeps the population array in scratch
+  // memory in every kernel that contains this operator.  Each face reads only
+  // the populations it does not write, so the values are those of the
+  // branch-per-face form, operation for operation.
   const bool at_max = !east;
   const T two = T(2.0);
   const T S0 = ((((((((f[0] + f[3]) + f[6]) + f[9]) + f[12]) + f[15]) + f[18]) + f[21]) + f[24]);
   const T y0 = (((((-f[3]) + f[6]) - f[12]) + f[15]) - f[21]) + f[24];
   const T z0 = (((((-f[9]) - f[12]) - f[15]) + f[18]) + f[21]) + f[24];
-  T ux, uy, uz;
-  if (at_max) {
-    const T k = ((((((((f[2] + f[5]) + f[8]) + f[11]) + f[14]) + f[17]) + f[20]) + f[23]) + f[26]);
-    ux = (S0 + two * k) - rs;
-    uy = y0 + two * (((((-f[5]) + f[8]) - f[14]) + f[17]) - f[23] + f[26]);
-    uz = z0 + two * (((((-f[11]) - f[14]) - f[17]) + f[20]) + f[23] + f[26]);
-  } else {
-    const T k = ((((((((f[1] + f[4]) + f[7]) + f[10]) + f[13]) + f[16]) + f[19]) + f[22]) + f[25]);
-    ux = rs - (S0 + two * k);
-    uy = y0 + two * (((((-f[4]) + f[7]) - f[13]) + f[16]) - f[22] + f[25]);
-    uz = z0 + two * (((((-f[10]) - f[13]) - f[16]) + f[19]) + f[22] + f[25]);
-  }
+  const T kmax = ((((((((f[2] + f[5]) + f[8]) + f[11]) + f[14]) + f[17]) + f[20]) + f[23]) + f[26]);
+  const T kmin = ((((((((f[1] + f[4]) + f[7]) + f[10]) + f[13]) + f[16]) + f[19]) + f[22]) + f[25]);
+  const T ux = at_max ? (S0 + two * kmax) - rs : rs - (S0 + two * kmin);
+  T uy = at_max ? y0 + two * (((((-f[5]) + f[8]) - f[14]) + f[17]) - f[23] + f[26])
+                : y0 + two * (((((-f[4]) + f[7]) - f[13]) + f[16]) - f[22] + f[25]);
+  T uz = at_max ? z0 + two * (((((-f[11]) - f[14]) - f[17]) + f[20]) + f[23] + f[26])
+                : z0 + two * (((((-f[10]) - f[13]) - f[16]) + f[19]) + f[22] + f[25]);
   if (normal_only) uy = uz = T(0);
   const T c49 = T(4.0 / 9.0), c19 = T(1.0 / 9.0), c136 = T(1.0 / 36.0);
-  if (at_max) {
-    f[1] = f[2] - c49 * ux;
-    f[4] = f[5] - c19 * (ux + uy);
-    f[7] = f[8] + c19 * ((-ux) + uy);
-    f[10] = f[11] - c19 * (ux + uz);
-    f[13] = f[14] - c136 * ((ux + uy) + uz);
-    f[16] = f[17] + c136 * (((-ux) + uy) + uz);
-    f[19] = f[20] + c19 * ((-ux) + uz);
-    f[22] = f[23] + c136 * (((-ux) - uy) + uz);
-    f[25] = f[26] + c136 * (((-ux) + uy) + uz);
-  } else {
-    f[2] = f[1] + c49 * ux;
-    f[5] = f[4] + c19 * (ux - uy);
-    f[8] = f[7] + c19 * (ux + uy);
-    f[11] = f[10] + c19 * (ux - uz);
-    f[14] = f[13] + c136 * ((ux - uy) - uz);
-    f[17] = f[16] + c136 * ((ux + uy) - uz);
-    f[20] = f[19] + c19 * (ux + uz);
-    f[23] = f[22] + c136 * ((ux - uy) + uz);
-    f[26] = f[25] + c136 * ((ux + uy) + uz);
-  }
+  // x = max face: the populations entering along -x (odd indices)
+  const T m1 = f[2] - c49 * ux;
+  const T m4 = f[5] - c19 * (ux + uy);
+  const T m7 = f[8] + c19 * ((-ux) + uy);
+  const T m10 = f[11] - c19 * (ux + uz);
+  const T m13 = f[14] - c136 * ((ux + uy) + uz);
+  const T m16 = f[17] + c136 * (((-ux) + uy) + uz);
+  const T m19 = f[20] + c19 * ((-ux) + uz);
+  const T m22 = f[23] + c136 * (((-ux) - uy) + uz);
+  const T m25 = f[26] + c136 * (((-ux) + uy) + uz);
+  // x = 0 face: the populations entering along +x (even indices)
+  const T p2 = f[1] + c49 * ux;
+  const T p5 = f[4] + c19 * (ux - uy);
+  const T p8 = f[7] + c19 * (ux + uy);
+  const T p11 = f[10] + c19 * (ux - uz);
+  const T p14 = f[13] + c136 * ((ux - uy) - uz);
+  const T p17 = f[16] + c136 * ((ux + uy) - uz);
+  const T p20 = f[19] + c19 * (ux + uz);
+  const T p23 = f[22] + c136 * ((ux - uy) + uz);
+  const T p26 = f[25] + c136 * ((ux + uy) + uz);
+  f[1] = at_max ? m1 : f[1];
+  f[4] = at_max ? m4 : f[4];
+  f[7] = at_max ? m7 : f[7];
+  f[10] = at_max ? m10 : f[10];
+  f[13] = at_max ? m13 : f[13];
+  f[16] = at_max ? m16 : f[16];
+  f[19] = at_max ? m19 : f[19];
+  f[22] = at_max ? m22 : f[22];
+  f[25] = at_max ? m25 : f[25];
+  f[2] = at_max ? f[2] : p2;
+  f[5] = at_max ? f[5] : p5;
+  f[8] = at_max ? f[8] : p8;
+  f[11] = at_max ? f[11] : p11;
+  f[14] = at_max ? f[14] : p14;
+  f[17] = at_max ? f[17] : p17;
+  f[20] = at_max ? f[20] : p20;
+  f[23] = at_max ? f[23] : p23;
+  f[26] = at_max ? f[26] : p26;
 }
 
 // ZouHeVelocityX<East>, D2Q9 index-hard-coded (boundary.hpp:769-803)

@@ -384,35 +384,39 @@ template <class S, class T>
 constexpr int aux_odd_st() { return (sizeof(S) == 4 && sizeof(T) == 4 && (LBMX_NT & 2)) ? 2 : 0; }
 
 // Minimum waves per SIMD asked of the compiler for the fused step, per
-// lattice and compute type (1: no request).  D3Q27 FP32 phase 0 takes 182
-// VGPRs (2 waves per SIMD) unconstrained; held to 168 (3 waves) with a few
-// spills it runs 5% faster (23.2 -> 24.4 GLUPS at 256^3).  Asking D3Q19 for
-// 4 waves fits it in 128 VGPRs without spills but gains nothing measurable.
+// lattice, compute type, phase and addressing form (1: no request).  D3Q27
+// FP32: phase 0 in the buffer form held to 4 waves (128 VGPRs), everything
+// else to 3 (phase 1 167 VGPRs, no spills): 256^3 phase 0 671 us against
+// 717 us at 3 waves; the global form at 4 waves spills (1250 us).  Asking
+// D3Q19 for 4 waves fits it in 128 VGPRs without spills but gains nothing.
 #ifndef LBMX_WAVES_Q19
 #define LBMX_WAVES_Q19 1
 #endif
 #ifndef LBMX_WAVES_Q27
-#define LBMX_WAVES_Q27 3
+#define LBMX_WAVES_Q27 4
+#endif
+#ifndef LBMX_WAVES_Q27_ODD
+#define LBMX_WAVES_Q27_ODD 3
 #endif
 // D2Q9 FP32: 8 waves (64 VGPRs) with the buffer form (phase 0 then needs 60
 // and no spills): the C3 particle step 1.5% faster; 8 waves alone, no change
 #ifndef LBMX_WAVES_Q9
 #define LBMX_WAVES_Q9 8
 #endif
-template <class L, class T>
+template <class L, class T, bool ODD, bool BUF>
 struct StepWaves {
   static constexpr int value = 1;
 };
-template <>
-struct StepWaves<D3Q19, float> {
+template <bool ODD, bool BUF>
+struct StepWaves<D3Q19, float, ODD, BUF> {
   static constexpr int value = LBMX_WAVES_Q19;
 };
-template <>
-struct StepWaves<D3Q27, float> {
-  static constexpr int value = LBMX_WAVES_Q27;
+template <bool ODD, bool BUF>
+struct StepWaves<D3Q27, float, ODD, BUF> {
+  static constexpr int value = (ODD || !BUF) ? LBMX_WAVES_Q27_ODD : LBMX_WAVES_Q27;
 };
-template <>
-struct StepWaves<D2Q9, float> {
+template <bool ODD, bool BUF>
+struct StepWaves<D2Q9, float, ODD, BUF> {
   static constexpr int value = LBMX_WAVES_Q9;
 };
 // ROW1: every block is one row segment (blockDim.y == 1), so y and z, the
@@ -420,7 +424,7 @@ struct StepWaves<D2Q9, float> {
 // (scalar registers) and each population access is a uniform row base plus
 // a 32-bit per-lane x offset; only x and its two neighbours live per lane.
 template <class L, class S, class T, bool WB, bool ODD, bool BUF, bool ROW1>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(StepWaves<L, T>::value))) void k_aa_step(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(StepWaves<L, T, ODD, BUF>::value))) void k_aa_step(
     StepArgs a) {
   constexpr int Q = L::Q;
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
