@@ -51,6 +51,13 @@ struct Act {  // element (pixel q, channel c) at p[q*ld + off + c]
 };
 
 constexpr int BM = 128, BK = 32, LK = BK + 8;
+// Conv K steps whose global loads are in flight across whole K steps (a
+// register ring of that many slots), per tile width.  0: the next step's
+// loads are issued at the start of a step and stored to LDS at its end.  2
+// for the 64-, 80- and 128-channel tiles (their conv 4-7% faster); the 32-
+// and 96-channel tiles are as fast or faster with 0 (the ring's registers
+// cost the 96-channel tile a wave per SIMD), and 4 or more slow every tile.
+constexpr int conv_prefetch(int nc) { return (nc == 4 || nc == 5 || nc == 8) ? 2 : 0; }
 
 struct ConvArgs {
   int B, H, W;
@@ -102,6 +109,8 @@ template <int NC>
 __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
   constexpr int BNc = 16 * NC;
   constexpr int NB = (BNc * 4 + 255) / 256;  // B chunks of 8 per thread
+  constexpr int KP = conv_prefetch(NC);
+  constexpr int KS = KP > 0 ? KP : 1;  // ring slots
   // the two tile buffers; after the K loop the same LDS holds the block's
   // bf16 results for 16-byte stores (128 rows of BNc + 8 values)
   constexpr int kTiles = 2 * BM * LK + 2 * BNc * LK, kOut = BM * (BNc + 8);
@@ -124,8 +133,10 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
     ry[j] = r / a.W;
     rx[j] = r - ry[j] * a.W;
   }
-  uint4 ra[2], rw[NB];
-  auto gload = [&](int k0) {
+  // slot q of the ring holds the A and B chunks of one K step from its load
+  // to its LDS store
+  uint4 ra[KS][2], rw[KS][NB];
+  auto gload = [&](int q, int k0) {
     const int kk = k0 + kc;
     int tap = 0, c = 0;
     if (kk < K) {
@@ -134,22 +145,22 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
     }
 #pragma unroll
     for (int j = 0; j < 2; ++j)
-      ra[j] = (rv[j] && kk < K) ? load_tap(a.in, a.H, a.W, rb[j], ry[j], rx[j], tap, c) : make_uint4(0u, 0u, 0u, 0u);
+      ra[q][j] = (rv[j] && kk < K) ? load_tap(a.in, a.H, a.W, rb[j], ry[j], rx[j], tap, c) : make_uint4(0u, 0u, 0u, 0u);
 #pragma unroll
-    for (int q = 0; q < NB; ++q) {
-      const int i = tid + 256 * q;
+    for (int qq = 0; qq < NB; ++qq) {
+      const int i = tid + 256 * qq;
       const int n = n0 + (i >> 2), k = k0 + (i & 3) * 8;
-      rw[q] = (i < BNc * 4 && n < N && k < K) ? *reinterpret_cast<const uint4*>(a.w + size_t(n) * K + k)
-                                               : make_uint4(0u, 0u, 0u, 0u);
+      rw[q][qq] = (i < BNc * 4 && n < N && k < K) ? *reinterpret_cast<const uint4*>(a.w + size_t(n) * K + k)
+                                                   : make_uint4(0u, 0u, 0u, 0u);
     }
   };
-  auto sstore = [&](int buf) {
+  auto sstore = [&](int buf, int q) {
 #pragma unroll
-    for (int j = 0; j < 2; ++j) *reinterpret_cast<uint4*>(&As[buf][((tid + 256 * j) >> 2) * LK + kc]) = ra[j];
+    for (int j = 0; j < 2; ++j) *reinterpret_cast<uint4*>(&As[buf][((tid + 256 * j) >> 2) * LK + kc]) = ra[q][j];
 #pragma unroll
-    for (int q = 0; q < NB; ++q) {
-      const int i = tid + 256 * q;
-      if (i < BNc * 4) *reinterpret_cast<uint4*>(&Bs[buf][(i >> 2) * LK + (i & 3) * 8]) = rw[q];
+    for (int qq = 0; qq < NB; ++qq) {
+      const int i = tid + 256 * qq;
+      if (i < BNc * 4) *reinterpret_cast<uint4*>(&Bs[buf][(i >> 2) * LK + (i & 3) * 8]) = rw[q][qq];
     }
   };
   f32x4 acc[2][NC];
@@ -158,15 +169,36 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
 #pragma unroll
     for (int c = 0; c < NC; ++c) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
   const int nk = (K + BK - 1) / BK;
-  gload(0);
-  sstore(0);
+  gload(0, 0);
+  sstore(0, 0);
+#pragma unroll
+  for (int j = 1; j <= KP; ++j)
+    if (j < nk) gload(j % KS, j * BK);
   __syncthreads();
-  for (int kt = 0; kt < nk; ++kt) {
-    const int cur = kt & 1;
-    if (kt + 1 < nk) gload((kt + 1) * BK);
-    mma_tile_n<NC>(As[cur], Bs[cur], wave, lane, acc);
-    if (kt + 1 < nk) sstore(cur ^ 1);  // read by no wave since the last barrier
-    __syncthreads();
+  if constexpr (KP == 0) {
+    for (int kt = 0; kt < nk; ++kt) {
+      if (kt + 1 < nk) gload(0, (kt + 1) * BK);
+      mma_tile_n<NC>(As[kt & 1], Bs[kt & 1], wave, lane, acc);
+      if (kt + 1 < nk) sstore((kt + 1) & 1, 0);  // read by no wave since the last barrier
+      __syncthreads();
+    }
+  } else {
+    // step kt: MFMAs on LDS buffer kt & 1; step kt + 1 from the ring into
+    // the other buffer (read by no wave since the last barrier); its slot
+    // then takes the loads of step kt + 1 + KP (slots are compile-time
+    // indices)
+    for (int kt0 = 0; kt0 < nk; kt0 += KS) {
+#pragma unroll
+      for (int j = 0; j < KS; ++j) {
+        const int kt = kt0 + j;
+        if (kt < nk) {
+          mma_tile_n<NC>(As[kt & 1], Bs[kt & 1], wave, lane, acc);
+          if (kt + 1 < nk) sstore((kt + 1) & 1, (j + 1) % KS);
+          if (kt + 1 + KP < nk) gload((j + 1) % KS, (kt + 1 + KP) * BK);
+          __syncthreads();
+        }
+      }
+    }
   }
   if (a.outf) {  // FP32 result (the output layer)
 #pragma unroll
