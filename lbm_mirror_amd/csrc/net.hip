@@ -800,7 +800,7 @@ lbm_status conv(lbm_net* n, int li, int B, Act in, Act out, float* outf) {
 // (in > 0) when mask_in.
 lbm_status conv_bwd(lbm_net* n, int li, int B, Act in, Act dpre, Act dx, bool mask_in) {
   const Layer& L = n->layers[li];
-  const int H = H_(n, L.level), W = W_(n, L.level), M = B * H * W;
+  const int H = H_(n, L.level), W = W_(n, L.level);
   int splits, chunk;
   wgrad_shape(n, L, B, splits, chunk);
   WgradArgs wa{B, H, W, L.cin, L.cpad, in, dpre, n->partial, chunk};
