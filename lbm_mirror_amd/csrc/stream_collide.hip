@@ -263,9 +263,14 @@ __device__ __forceinline__ void collide_cell(T (&f)[L::Q], uint8_t fl, const Ste
   // When every active lane of the wavefront carries the same flag (the bulk
   // of any lattice), the operator is read with scalar loads and its kind is
   // a scalar branch; mixed wavefronts take the per-lane path.
-  // the operator table is global memory: typed so, its per-lane reads are
-  // GLOBAL loads also where the table pointer itself was loaded from memory
-  using GOp = const __attribute__((address_space(1))) OpDev;
+  // The operator table is written only by the host between launches: typed
+  // as constant memory, a wavefront-uniform operator is read with scalar
+  // loads whatever the kernel stores before its fields are used (as global
+  // memory, the fault-word store of the macro-writing branch made the
+  // compiler read every operator field with vector loads and wait for them:
+  // the D3Q19 phase 1 205 -> 240 us); per-lane reads are vector loads either
+  // way.
+  using GOp = const __attribute__((address_space(4))) OpDev;
   GOp* ops = (GOp*)a.ops;
   const int fl0 = __builtin_amdgcn_readfirstlane(int(fl));
   if (__all(int(fl) == fl0)) {
